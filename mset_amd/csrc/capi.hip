@@ -1,0 +1,898 @@
+// capi.hip -- host engine behind include/mset.h.
+//
+// mset_ctx holds the device state of one tilt (the role of mset::paras, src/CUDA/mset.cuh:23-115, and
+// of the module-static shared_ptr of src/CUDA/mex_main.cu:13).  The large per-workgroup scratch of
+// the batched kernels lives in a Workspace that the tilts of one reconstruction share.  mset_recon
+// keeps a whole tilt series resident and runs the update and error passes of
+// src/Run_MSET_recon.m:119-239 on the device, optionally replaying a captured hipGraph.
+// Every HIP call is checked; failures become status codes plus a thread-local message instead of
+// mexErrMsgTxt or unchecked calls.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/mset.h"
+#include "kernels.hpp"
+#include "rotate.hpp"
+
+using namespace mset;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                           \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail(e_ == hipErrorOutOfMemory ? MSET_ENOMEM : MSET_EDEVICE,                \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                    \
+    } while (0)
+
+#define CHK(expr)                         \
+    do {                                  \
+        int r_ = (expr);                  \
+        if (r_ != MSET_OK) return r_;     \
+    } while (0)
+
+template <typename T> struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    int ensure(size_t count, const char* what) {
+        if (count <= n && p) return MSET_OK;
+        release();
+        if (count == 0) return MSET_OK;
+        hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(MSET_ENOMEM, std::string("hipMalloc ") + what + ": " + hipGetErrorString(e));
+        }
+        n = count;
+        return MSET_OK;
+    }
+};
+
+struct Prof {
+    int launches = 0;
+    double ms = 0.0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+};
+using ProfMap = std::map<std::string, Prof>;
+
+// scratch shared by the tilts of one reconstruction
+struct Workspace {
+    DevBuf<float2> save, trans, probeF, pslab, PG;
+    DevBuf<float> gslab, G;
+    RotWork rot;
+    ~Workspace() { rot_release(rot); }
+};
+
+}  // namespace
+
+struct mset_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = true;
+    int cus = 0;
+    mset_params p{};
+    int n = 0, nz = 0, L = 0, npos = 0;
+    bool uploaded = false;
+    DevBuf<float> V, sqrtM, x, y, dpos, Mtmp;
+    DevBuf<float2> probe, prop, tw;
+    DevBuf<double> err;
+    std::shared_ptr<Workspace> ws;
+    int slots = 0;
+    bool* profiling = nullptr;
+    std::shared_ptr<ProfMap> prof;
+    bool own_profiling = false;
+};
+
+namespace {
+
+// event bracket around a launch when profiling is on (never inside a graph capture)
+struct Timed {
+    hipStream_t st;
+    ProfMap* pm;
+    const char* name;
+    hipEvent_t a = nullptr, b = nullptr;
+    Timed(mset_ctx* c, const char* n_) : st(c->stream), pm(*c->profiling ? c->prof.get() : nullptr), name(n_) {
+        if (pm) {
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            (void)hipEventRecord(a, st);
+        }
+    }
+    ~Timed() {
+        if (pm) {
+            (void)hipEventRecord(b, st);
+            (*pm)[name].pending.emplace_back(a, b);
+        }
+    }
+};
+
+int prof_read(ProfMap& pm, hipStream_t st, const char* name, int* launches, double* total_ms) {
+    HIPCHK(hipStreamSynchronize(st));
+    Prof& P = pm[name];
+    for (auto& ev : P.pending) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, ev.first, ev.second));
+        P.ms += ms;
+        P.launches += 1;
+        (void)hipEventDestroy(ev.first);
+        (void)hipEventDestroy(ev.second);
+    }
+    P.pending.clear();
+    if (launches) *launches = P.launches;
+    if (total_ms) *total_ms = P.ms;
+    P.launches = 0;
+    P.ms = 0.0;
+    return MSET_OK;
+}
+
+void prof_free(ProfMap& pm) {
+    for (auto& kv : pm)
+        for (auto& ev : kv.second.pending) {
+            (void)hipEventDestroy(ev.first);
+            (void)hipEventDestroy(ev.second);
+        }
+    pm.clear();
+}
+
+int check_ctx(mset_ctx* c) {
+    if (!c) return fail(MSET_EINVAL, "null context");
+    HIPCHK(hipSetDevice(c->device));
+    return MSET_OK;
+}
+
+int check_uploaded(mset_ctx* c) {
+    CHK(check_ctx(c));
+    if (!c->uploaded) return fail(MSET_EINVAL, "no data uploaded (call mset_upload first)");
+    return MSET_OK;
+}
+
+int check_method(int method) {
+    if (method == MSET_METHOD_SSET)
+        return fail(MSET_EUNSUPPORTED, "the SSET method (run_sset/error_sset) is not provided by this engine");
+    if (method != MSET_METHOD_MSET) return fail(MSET_EINVAL, "method must be MSET_METHOD_MSET");
+    return MSET_OK;
+}
+
+void twiddles(int n, std::vector<float2>& tw) {
+    tw.resize(n);
+    for (int k = 0; k < n; ++k) {
+        const long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)n;
+        tw[k] = make_float2((float)cosl(a), (float)sinl(a));
+    }
+}
+
+size_t plane(const mset_ctx* c) { return (size_t)c->n * c->n; }
+
+PosArgs base_args(mset_ctx* c) {
+    PosArgs A{};
+    A.nz = c->nz;
+    A.L = c->L;
+    A.bin = c->p.slice_binning;
+    A.sigma = c->p.sigma;
+    A.dx = c->p.potential_pixelsize;
+    A.cent = (float)std::floor((c->n + 1) / 2.0 + 0.5);
+    A.s1 = c->p.step_size[0];
+    A.s2 = c->p.step_size[1];
+    A.s3 = c->p.step_size[2];
+    A.grad_mode = c->p.grad_mode;
+    A.want = (c->p.step_size[1] != 0.f ? WANT_PROBE : 0) | (c->p.step_size[2] != 0.f ? WANT_POS : 0);
+    A.tw = c->tw.p;
+    A.prop = c->prop.p;
+    A.sqrtM = c->sqrtM.p;
+    A.V = c->V.p;
+    A.trans = c->ws->trans.p;
+    A.probe = c->probe.p;
+    A.probeF = c->ws->probeF.p;
+    A.xlist = c->x.p;
+    A.ylist = c->y.p;
+    A.err_pos = c->err.p;
+    A.save = c->ws->save.p;
+    A.gslab = c->ws->gslab.p;
+    A.pslab = c->ws->pslab.p;
+    A.dpos = c->dpos.p;
+    return A;
+}
+
+// workgroups that can be resident at once for the batched kernels (1 per CU at N = 128)
+int batch_slots(mset_ctx* c) {
+    int per_cu = 1;
+    if (occupancy_batch(c->n, &per_cu) != hipSuccess || per_cu < 1) per_cu = 1;
+    int s = c->cus * per_cu;
+    if (s > c->npos) s = c->npos;
+    return s < 1 ? 1 : s;
+}
+
+int ensure_fixed_buffers(mset_ctx* c) {
+    Workspace& W = *c->ws;
+    CHK(W.trans.ensure(plane(c) * c->L, "transmission"));
+    CHK(W.probeF.ensure(plane(c), "probe spectrum"));
+    return MSET_OK;
+}
+
+int ensure_batch_buffers(mset_ctx* c) {
+    Workspace& W = *c->ws;
+    const int s = batch_slots(c);
+    CHK(ensure_fixed_buffers(c));
+    CHK(W.save.ensure((size_t)s * (c->L + 2) * plane(c), "save waves"));
+    CHK(W.gslab.ensure((size_t)s * c->L * plane(c), "gradient slabs"));
+    CHK(W.pslab.ensure((size_t)s * plane(c), "probe slabs"));
+    CHK(W.G.ensure((size_t)c->L * plane(c), "gradient"));
+    CHK(W.PG.ensure(plane(c), "probe gradient"));
+    CHK(c->dpos.ensure((size_t)2 * c->npos, "position terms"));
+    c->slots = s;
+    return MSET_OK;
+}
+
+// transmission planes + fftn(probe) for the fixed-state kernels
+int prepare_fixed(mset_ctx* c) {
+    CHK(ensure_fixed_buffers(c));
+    HIPCHK(launch_transmission(c->V.p, c->ws->trans.p, c->n, c->nz, c->p.slice_binning, c->p.sigma,
+                               c->p.potential_pixelsize, c->stream));
+    HIPCHK(launch_plane_fft(c->n, c->probe.p, c->ws->probeF.p, c->tw.p, 0, 1.f, 0.f, c->stream));
+    return MSET_OK;
+}
+
+double host_error_sum(const mset_ctx* c, const std::vector<double>& e) {
+    double s = 0.0;
+    for (double v : e) s += v;
+    return s / ((double)c->p.n_scan_x * (double)c->p.n_scan_y) / (double)plane(c);
+}
+
+int read_errors(mset_ctx* c, int first, int count, double* out) {
+    std::vector<double> e(count);
+    HIPCHK(hipMemcpyAsync(e.data(), c->err.p + first, sizeof(double) * count, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *out = host_error_sum(c, e);
+    return MSET_OK;
+}
+
+// one launch per position, sequential updates (issue only; no host synchronisation)
+int issue_plain(mset_ctx* c) {
+    CHK(c->ws->save.ensure((size_t)(c->L + 2) * plane(c), "save waves"));
+    PosArgs A = base_args(c);
+    for (int k = 0; k < c->npos; ++k) {
+        Timed t(c, "position_plain");
+        HIPCHK(launch_plain(c->n, A, k, c->stream));
+    }
+    return MSET_OK;
+}
+
+int issue_fwd(mset_ctx* c, float* inten) {
+    CHK(prepare_fixed(c));
+    PosArgs A = base_args(c);
+    A.first = 0;
+    A.count = c->npos;
+    A.inten = inten;
+    Timed t(c, "position_fwd");
+    HIPCHK(launch_fwd(c->n, A, batch_slots(c), c->stream));
+    return MSET_OK;
+}
+
+// gradient sums at the fixed state over [first, first+count) into W.G (L planes), W.PG (probe step
+// direction, real space) and c->dpos; with apply, the steps are taken (RVol += s1 G, probe -= s2 PG,
+// positions += s3 dpos).  want: the terms to form (the gradient request forms all of them).
+int issue_grad(mset_ctx* c, int first, int count, bool apply, int want) {
+    CHK(ensure_batch_buffers(c));
+    CHK(prepare_fixed(c));
+    Workspace& W = *c->ws;
+    const int grid = std::min(c->slots, count);
+    PosArgs A = base_args(c);
+    A.first = first;
+    A.count = count;
+    A.want = want;
+    HIPCHK(launch_fill(W.gslab.p, sizeof(float) * (size_t)grid * c->L * plane(c), c->stream));
+    if (want & WANT_PROBE) HIPCHK(launch_fill(W.pslab.p, sizeof(float2) * (size_t)grid * plane(c), c->stream));
+    {
+        Timed t(c, "position_batch");
+        HIPCHK(launch_batch(c->n, A, grid, c->stream));
+    }
+    HIPCHK(launch_reduce_slabs(W.gslab.p, grid, (size_t)c->L * plane(c), W.G.p, c->stream));
+    if (want & WANT_PROBE) {
+        // in place into slab 0 (each element is read and written by the same thread), then PG = ifftn
+        HIPCHK(launch_reduce_cslabs(W.pslab.p, grid, plane(c), W.pslab.p, c->stream));
+        HIPCHK(launch_plane_fft(c->n, W.pslab.p, W.PG.p, c->tw.p, 1, 1.f, 0.f, c->stream));
+    }
+    if (apply) {
+        if (c->p.step_size[0] != 0.f)
+            HIPCHK(launch_update_volume(c->V.p, W.G.p, c->n, c->nz, c->p.slice_binning, c->p.step_size[0], c->stream));
+        if (c->p.step_size[1] != 0.f && (want & WANT_PROBE))
+            HIPCHK(launch_plane_fft(c->n, W.pslab.p, c->probe.p, c->tw.p, 1, -c->p.step_size[1], 1.f, c->stream));
+        if (c->p.step_size[2] != 0.f && (want & WANT_POS))
+            HIPCHK(launch_update_positions(c->x.p + first, c->y.p + first, c->dpos.p + 2 * (size_t)first, count,
+                                           c->p.step_size[2], c->stream));
+    }
+    return MSET_OK;
+}
+
+int want_from_steps(const mset_ctx* c) {
+    return (c->p.step_size[1] != 0.f ? WANT_PROBE : 0) | (c->p.step_size[2] != 0.f ? WANT_POS : 0);
+}
+
+int issue_batched(mset_ctx* c, int batch) {
+    for (int first = 0; first < c->npos; first += batch) {
+        const int cnt = std::min(batch, c->npos - first);
+        CHK(issue_grad(c, first, cnt, true, want_from_steps(c)));
+    }
+    return MSET_OK;
+}
+
+int new_ctx(mset_ctx** out, int device, hipStream_t shared_stream, std::shared_ptr<Workspace> ws,
+            bool* profiling, std::shared_ptr<ProfMap> prof) {
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    mset_ctx* c = new mset_ctx();
+    c->device = device;
+    c->cus = prop.multiProcessorCount;
+    if (shared_stream) {
+        c->stream = shared_stream;
+        c->own_stream = false;
+    } else {
+        hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete c;
+            return fail(MSET_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+        }
+    }
+    c->ws = ws ? ws : std::make_shared<Workspace>();
+    if (profiling) {
+        c->profiling = profiling;
+    } else {
+        c->profiling = new bool(false);
+        c->own_profiling = true;
+    }
+    c->prof = prof ? prof : std::make_shared<ProfMap>();
+    *out = c;
+    return MSET_OK;
+}
+
+void free_ctx(mset_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    if (c->prof.use_count() == 1) prof_free(*c->prof);
+    if (c->own_stream) (void)hipStreamDestroy(c->stream);
+    if (c->own_profiling) delete c->profiling;
+    delete c;
+}
+
+// validate + set geometry + copy a tilt's inputs (MEX 'upload', mex_main.cu:42-95)
+int upload_impl(mset_ctx* c, const mset_params* p, const float* RVol, const float* init_wave2D, const float* prop2D,
+                const float* xlist, const float* ylist, const float* measured) {
+    if (!p || !init_wave2D || !prop2D || !xlist || !ylist || !measured) return fail(MSET_EINVAL, "null input array");
+    if (p->nx != p->ny) return fail(MSET_EINVAL, "RVol must be square in x, y (nx == ny)");
+    if (p->nx <= 0 || p->nz <= 0 || p->num_scan_pos <= 0 || p->n_scan_x <= 0 || p->n_scan_y <= 0)
+        return fail(MSET_EINVAL, "sizes must be positive");
+    if (!plane_size_supported(p->nx))
+        return fail(MSET_EUNSUPPORTED, "lateral size " + std::to_string(p->nx) +
+                                           " not built (supported: 16, 24, 32, 48, 64, 96, 128)");
+    if (p->slice_binning <= 0) return fail(MSET_EINVAL, "slice_binning must be a positive integer");
+    if (p->grad_mode != MSET_GRAD_REF && p->grad_mode != MSET_GRAD_EXACT) return fail(MSET_EINVAL, "bad grad_mode");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->uploaded = false;
+    c->p = *p;
+    if (c->p.slice_binning > p->nz) c->p.slice_binning = p->nz;   // STEP00_INIT.m:127-129
+    c->n = p->nx;
+    c->nz = p->nz;
+    c->L = (p->nz + c->p.slice_binning - 1) / c->p.slice_binning;
+    c->npos = p->num_scan_pos;
+    const size_t n2 = plane(c);
+    CHK(c->V.ensure(n2 * c->nz, "volume"));
+    CHK(c->probe.ensure(n2, "probe"));
+    CHK(c->prop.ensure(n2, "propagator"));
+    CHK(c->tw.ensure(c->n, "twiddles"));
+    CHK(c->sqrtM.ensure(n2 * c->npos, "patterns"));
+    CHK(c->x.ensure(c->npos, "scan x"));
+    CHK(c->y.ensure(c->npos, "scan y"));
+    CHK(c->err.ensure(c->npos, "errors"));
+    CHK(c->Mtmp.ensure(n2 * c->npos, "pattern staging"));
+
+    // ifftshift of the MATLAB-layout propagator (mex_main.cu:80)
+    std::vector<float2> pu(n2);
+    const int n = c->n, h = n / 2;
+    for (int y = 0; y < n; ++y)
+        for (int xx = 0; xx < n; ++xx) {
+            const size_t src = (size_t)((y + h) % n) * n + (xx + h) % n;
+            pu[(size_t)y * n + xx] = make_float2(prop2D[2 * src], prop2D[2 * src + 1]);
+        }
+    std::vector<float2> tw;
+    twiddles(n, tw);
+    if (RVol) HIPCHK(hipMemcpyAsync(c->V.p, RVol, sizeof(float) * n2 * c->nz, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->probe.p, init_wave2D, sizeof(float2) * n2, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->prop.p, pu.data(), sizeof(float2) * n2, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->tw.p, tw.data(), sizeof(float2) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->x.p, xlist, sizeof(float) * c->npos, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->y.p, ylist, sizeof(float) * c->npos, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->Mtmp.p, measured, sizeof(float) * n2 * c->npos, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(launch_ingest_patterns(c->Mtmp.p, c->sqrtM.p, n, c->npos, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->Mtmp.release();
+    c->uploaded = true;
+    return MSET_OK;
+}
+
+int rotate_volume(mset_ctx* c, float* vol, const double* v1, const double* v2, const double* v3, const double* ang,
+                  bool inverse) {
+    double R[9];
+    if (!rot_matrix(v1, v2, v3, ang[0], ang[1], ang[2], inverse, R)) return MSET_OK;   // |angles| <= 0.5
+    if (!rot_size_supported(c->n) || !rot_size_supported(c->nz))
+        return fail(MSET_EUNSUPPORTED, "rotation line length not built");
+    Timed t(c, "rotate");
+    const int r = rot_run(c->ws->rot, vol, c->n, c->n, c->nz, R, 11.0, c->stream);
+    if (r != MSET_OK) return fail(r, rot_last_error());
+    return MSET_OK;
+}
+
+}  // namespace
+
+// ================================================================================================
+struct mset_recon {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::shared_ptr<Workspace> ws;
+    std::shared_ptr<ProfMap> prof;
+    bool profiling = false;
+    mset_params common{};
+    int n = 0, nz = 0;
+    double vec[9] = {0, 0, 1, 0, 1, 0, 1, 0, 0};
+    DevBuf<float> rec, rec_snap;
+    std::vector<float> prop_host;
+    std::vector<mset_ctx*> tilts;
+    std::vector<std::array<double, 3>> angles;
+    std::vector<std::vector<float>> snap_probe, snap_x, snap_y;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int graph_key[3] = {-1, -1, -1};
+};
+
+namespace {
+
+void drop_graph(mset_recon* r) {
+    if (r->exec) (void)hipGraphExecDestroy(r->exec);
+    if (r->graph) (void)hipGraphDestroy(r->graph);
+    r->exec = nullptr;
+    r->graph = nullptr;
+    r->graph_key[0] = r->graph_key[1] = r->graph_key[2] = -1;
+}
+
+int check_recon(mset_recon* r) {
+    if (!r) return fail(MSET_EINVAL, "null reconstruction");
+    HIPCHK(hipSetDevice(r->device));
+    for (size_t j = 0; j < r->tilts.size(); ++j)
+        if (!r->tilts[j] || !r->tilts[j]->uploaded)
+            return fail(MSET_EINVAL, "tilt " + std::to_string(j) + " has no data (call mset_recon_set_tilt)");
+    return MSET_OK;
+}
+
+// one update pass over every tilt (Run_MSET_recon.m:119-187), issue only
+int issue_update_pass(mset_recon* r, int path, int batch, int positivity) {
+    const size_t nv = (size_t)r->n * r->n * r->nz;
+    for (size_t j = 0; j < r->tilts.size(); ++j) {
+        mset_ctx* c = r->tilts[j];
+        // STEP01: RVol = inv_rot(rec)
+        HIPCHK(hipMemcpyAsync(c->V.p, r->rec.p, sizeof(float) * nv, hipMemcpyDeviceToDevice, r->stream));
+        CHK(rotate_volume(c, c->V.p, r->vec, r->vec + 3, r->vec + 6, r->angles[j].data(), true));
+        // STEP02-04
+        if (path == MSET_PATH_PLAIN) CHK(issue_plain(c));
+        else CHK(issue_batched(c, batch));
+        // STEP05: rec = rot(RVol), positivity
+        HIPCHK(hipMemcpyAsync(r->rec.p, c->V.p, sizeof(float) * nv, hipMemcpyDeviceToDevice, r->stream));
+        CHK(rotate_volume(c, r->rec.p, r->vec, r->vec + 3, r->vec + 6, r->angles[j].data(), false));
+        if (positivity) HIPCHK(launch_positivity(r->rec.p, nv, r->stream));
+    }
+    return MSET_OK;
+}
+
+}  // namespace
+
+// ================================================================================================
+extern "C" {
+
+const char* mset_last_error(void) { return g_err.c_str(); }
+
+int mset_create(mset_ctx** out, int device) {
+    if (!out) return fail(MSET_EINVAL, "null output pointer");
+    *out = nullptr;
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(MSET_EINVAL, "device index out of range");
+    HIPCHK(hipSetDevice(device));
+    return new_ctx(out, device, nullptr, nullptr, nullptr, nullptr);
+}
+
+int mset_destroy(mset_ctx* c) {
+    free_ctx(c);
+    return MSET_OK;
+}
+
+void* mset_stream(mset_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int mset_upload(mset_ctx* c, const mset_params* p, const float* RVol, const float* init_wave2D, const float* prop2D,
+                const float* xlist, const float* ylist, const float* measured) {
+    CHK(check_ctx(c));
+    if (!RVol) return fail(MSET_EINVAL, "null input array");
+    return upload_impl(c, p, RVol, init_wave2D, prop2D, xlist, ylist, measured);
+}
+
+int mset_run_resident(mset_ctx* c, int method, double* err_out) {
+    CHK(check_uploaded(c));
+    CHK(check_method(method));
+    CHK(issue_plain(c));
+    if (err_out) return read_errors(c, 0, c->npos, err_out);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MSET_OK;
+}
+
+int mset_error_resident(mset_ctx* c, int method, double* err_out) {
+    CHK(check_uploaded(c));
+    CHK(check_method(method));
+    if (!err_out) return fail(MSET_EINVAL, "null error output");
+    CHK(issue_fwd(c, nullptr));
+    return read_errors(c, 0, c->npos, err_out);
+}
+
+int mset_simulate(mset_ctx* c, float* intensity_out) {
+    CHK(check_uploaded(c));
+    if (!intensity_out) return fail(MSET_EINVAL, "null output");
+    DevBuf<float> I;
+    CHK(I.ensure(plane(c) * c->npos, "intensities"));
+    CHK(issue_fwd(c, I.p));
+    HIPCHK(hipMemcpyAsync(intensity_out, I.p, sizeof(float) * plane(c) * c->npos, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MSET_OK;
+}
+
+int mset_download(mset_ctx* c, float* RVol, float* probe, float* xl, float* yl) {
+    CHK(check_uploaded(c));
+    const size_t n2 = plane(c);
+    if (RVol) HIPCHK(hipMemcpyAsync(RVol, c->V.p, sizeof(float) * n2 * c->nz, hipMemcpyDeviceToHost, c->stream));
+    if (probe) HIPCHK(hipMemcpyAsync(probe, c->probe.p, sizeof(float2) * n2, hipMemcpyDeviceToHost, c->stream));
+    if (xl) HIPCHK(hipMemcpyAsync(xl, c->x.p, sizeof(float) * c->npos, hipMemcpyDeviceToHost, c->stream));
+    if (yl) HIPCHK(hipMemcpyAsync(yl, c->y.p, sizeof(float) * c->npos, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MSET_OK;
+}
+
+int mset_set_state(mset_ctx* c, const float* RVol, const float* probe, const float* xl, const float* yl) {
+    CHK(check_uploaded(c));
+    const size_t n2 = plane(c);
+    if (RVol) HIPCHK(hipMemcpyAsync(c->V.p, RVol, sizeof(float) * n2 * c->nz, hipMemcpyHostToDevice, c->stream));
+    if (probe) HIPCHK(hipMemcpyAsync(c->probe.p, probe, sizeof(float2) * n2, hipMemcpyHostToDevice, c->stream));
+    if (xl) HIPCHK(hipMemcpyAsync(c->x.p, xl, sizeof(float) * c->npos, hipMemcpyHostToDevice, c->stream));
+    if (yl) HIPCHK(hipMemcpyAsync(c->y.p, yl, sizeof(float) * c->npos, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MSET_OK;
+}
+
+int mset_run(mset_ctx* c, int method, float* RVol_out, float* probe_out, float* xl_out, float* yl_out,
+             float* err_out) {
+    double e = 0.0;
+    CHK(mset_run_resident(c, method, err_out ? &e : nullptr));
+    CHK(mset_download(c, RVol_out, probe_out, xl_out, yl_out));
+    if (err_out) *err_out = (float)e;
+    return MSET_OK;
+}
+
+int mset_error(mset_ctx* c, int method, float* err_out) {
+    if (!err_out) return fail(MSET_EINVAL, "null error output");
+    double e = 0.0;
+    CHK(mset_error_resident(c, method, &e));
+    *err_out = (float)e;
+    return MSET_OK;
+}
+
+int mset_grad(mset_ctx* c, int first, int count, float* dV, float* dprobe, float* dpos, double* err_out) {
+    CHK(check_uploaded(c));
+    if (first < 0 || count <= 0 || first + count > c->npos) return fail(MSET_EINVAL, "position range out of bounds");
+    CHK(issue_grad(c, first, count, false, WANT_PROBE | WANT_POS));
+    const size_t n2 = plane(c);
+    if (dV) HIPCHK(hipMemcpyAsync(dV, c->ws->G.p, sizeof(float) * n2 * c->L, hipMemcpyDeviceToHost, c->stream));
+    if (dprobe) HIPCHK(hipMemcpyAsync(dprobe, c->ws->PG.p, sizeof(float2) * n2, hipMemcpyDeviceToHost, c->stream));
+    if (dpos) HIPCHK(hipMemcpyAsync(dpos, c->dpos.p + 2 * (size_t)first, sizeof(float) * 2 * count,
+                                    hipMemcpyDeviceToHost, c->stream));
+    if (err_out) return read_errors(c, first, count, err_out);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MSET_OK;
+}
+
+int mset_waves(mset_ctx* c, int k, float* waves_out) {
+    CHK(check_uploaded(c));
+    if (k < 0 || k >= c->npos || !waves_out) return fail(MSET_EINVAL, "bad position index or null output");
+    CHK(ensure_batch_buffers(c));
+    CHK(prepare_fixed(c));
+    PosArgs A = base_args(c);
+    A.first = k;
+    A.count = 1;
+    A.want = 0;
+    HIPCHK(launch_fill(c->ws->gslab.p, sizeof(float) * (size_t)c->L * plane(c), c->stream));
+    HIPCHK(launch_batch(c->n, A, 1, c->stream));
+    HIPCHK(hipMemcpyAsync(waves_out, c->ws->save.p, sizeof(float2) * (size_t)(c->L + 1) * plane(c),
+                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MSET_OK;
+}
+
+int mset_run_batched(mset_ctx* c, int method, int batch, double* err_out) {
+    CHK(check_uploaded(c));
+    CHK(check_method(method));
+    if (batch <= 0) return fail(MSET_EINVAL, "batch must be positive");
+    CHK(issue_batched(c, batch));
+    if (err_out) return read_errors(c, 0, c->npos, err_out);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MSET_OK;
+}
+
+int mset_rotate(mset_ctx* c, const float* in, float* out, int nx, int ny, int nz, const double* R, double step) {
+    CHK(check_ctx(c));
+    if (!in || !out || !R) return fail(MSET_EINVAL, "null argument");
+    if (nx <= 0 || ny <= 0 || nz <= 0) return fail(MSET_EINVAL, "sizes must be positive");
+    if (!rot_size_supported(nx) || !rot_size_supported(ny) || !rot_size_supported(nz))
+        return fail(MSET_EUNSUPPORTED, "rotation line length not built");
+    const size_t nv = (size_t)nx * ny * nz;
+    DevBuf<float> buf;
+    CHK(buf.ensure(nv, "rotation volume"));
+    HIPCHK(hipMemcpyAsync(buf.p, in, sizeof(float) * nv, hipMemcpyHostToDevice, c->stream));
+    {
+        Timed t(c, "rotate");
+        const int r = rot_run(c->ws->rot, buf.p, nx, ny, nz, R, step, c->stream);
+        if (r != MSET_OK) return fail(r, rot_last_error());
+    }
+    HIPCHK(hipMemcpyAsync(out, buf.p, sizeof(float) * nv, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MSET_OK;
+}
+
+int mset_rotate_resident(mset_ctx* c, const double* v1, const double* v2, const double* v3, double phi, double theta,
+                         double psi, int inverse) {
+    CHK(check_uploaded(c));
+    if (!v1 || !v2 || !v3) return fail(MSET_EINVAL, "null rotation axis");
+    const double ang[3] = {phi, theta, psi};
+    CHK(rotate_volume(c, c->V.p, v1, v2, v3, ang, inverse != 0));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MSET_OK;
+}
+
+int mset_profile_enable(mset_ctx* c, int enable) {
+    CHK(check_ctx(c));
+    *c->profiling = enable != 0;
+    return MSET_OK;
+}
+
+int mset_profile_read(mset_ctx* c, const char* name, int* launches, double* total_ms) {
+    CHK(check_ctx(c));
+    if (!name) return fail(MSET_EINVAL, "null name");
+    return prof_read(*c->prof, c->stream, name, launches, total_ms);
+}
+
+int mset_synchronize(mset_ctx* c) {
+    CHK(check_ctx(c));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MSET_OK;
+}
+
+// ---------------------------------------------------------------------------------------- recon
+int mset_recon_create(mset_recon** out, int device, const mset_params* common, int n_tilts, const double* vecs,
+                      const float* rec, const float* prop2D) {
+    if (!out || !common || !rec || !prop2D) return fail(MSET_EINVAL, "null argument");
+    *out = nullptr;
+    if (n_tilts <= 0) return fail(MSET_EINVAL, "n_tilts must be positive");
+    if (common->nx != common->ny || common->nx <= 0 || common->nz <= 0)
+        return fail(MSET_EINVAL, "volume must be square in x, y with positive sizes");
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(MSET_EINVAL, "device index out of range");
+    HIPCHK(hipSetDevice(device));
+    auto r = std::make_unique<mset_recon>();
+    r->device = device;
+    r->common = *common;
+    r->n = common->nx;
+    r->nz = common->nz;
+    if (vecs) std::memcpy(r->vec, vecs, sizeof(r->vec));
+    HIPCHK(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
+    r->ws = std::make_shared<Workspace>();
+    r->prof = std::make_shared<ProfMap>();
+    const size_t nv = (size_t)r->n * r->n * r->nz;
+    CHK(r->rec.ensure(nv, "reconstruction"));
+    HIPCHK(hipMemcpyAsync(r->rec.p, rec, sizeof(float) * nv, hipMemcpyHostToDevice, r->stream));
+    HIPCHK(hipStreamSynchronize(r->stream));
+    r->prop_host.assign(prop2D, prop2D + 2 * (size_t)r->n * r->n);
+    r->tilts.assign(n_tilts, nullptr);
+    r->angles.assign(n_tilts, {0, 0, 0});
+    *out = r.release();
+    return MSET_OK;
+}
+
+int mset_recon_destroy(mset_recon* r) {
+    if (!r) return MSET_OK;
+    (void)hipSetDevice(r->device);
+    (void)hipStreamSynchronize(r->stream);
+    drop_graph(r);
+    for (mset_ctx* c : r->tilts) free_ctx(c);
+    prof_free(*r->prof);
+    r->ws.reset();
+    (void)hipStreamDestroy(r->stream);
+    delete r;
+    return MSET_OK;
+}
+
+int mset_recon_set_tilt(mset_recon* r, int j, const double* angles, const float* probe, int n_scan_x, int n_scan_y,
+                        int num_scan_pos, const float* xlist, const float* ylist, const float* measured) {
+    if (!r) return fail(MSET_EINVAL, "null reconstruction");
+    HIPCHK(hipSetDevice(r->device));
+    if (j < 0 || j >= (int)r->tilts.size()) return fail(MSET_EINVAL, "tilt index out of range");
+    if (!angles) return fail(MSET_EINVAL, "null angles");
+    drop_graph(r);
+    if (!r->tilts[j]) CHK(new_ctx(&r->tilts[j], r->device, r->stream, r->ws, &r->profiling, r->prof));
+    mset_params p = r->common;
+    p.n_scan_x = n_scan_x;
+    p.n_scan_y = n_scan_y;
+    p.num_scan_pos = num_scan_pos;
+    CHK(upload_impl(r->tilts[j], &p, nullptr, probe, r->prop_host.data(), xlist, ylist, measured));
+    r->angles[j] = {angles[0], angles[1], angles[2]};
+    return MSET_OK;
+}
+
+int mset_recon_set_steps(mset_recon* r, const float* steps) {
+    if (!r || !steps) return fail(MSET_EINVAL, "null argument");
+    for (int i = 0; i < 3; ++i) r->common.step_size[i] = steps[i];
+    for (mset_ctx* c : r->tilts)
+        if (c)
+            for (int i = 0; i < 3; ++i) c->p.step_size[i] = steps[i];
+    drop_graph(r);
+    return MSET_OK;
+}
+
+int mset_recon_iterate(mset_recon* r, int path, int batch, int use_graph, int positivity, double* err_out) {
+    CHK(check_recon(r));
+    if (path != MSET_PATH_PLAIN && path != MSET_PATH_BATCHED) return fail(MSET_EINVAL, "bad path");
+    if (path == MSET_PATH_BATCHED && batch <= 0) return fail(MSET_EINVAL, "batch must be positive");
+    if (path == MSET_PATH_PLAIN) batch = 1;
+    if (use_graph && !r->profiling) {
+        const bool same = r->exec && r->graph_key[0] == path && r->graph_key[1] == batch && r->graph_key[2] == positivity;
+        if (!same) {
+            drop_graph(r);
+            // eager pass first: every allocation and twiddle upload happens outside the capture
+            CHK(issue_update_pass(r, path, batch, positivity));
+            HIPCHK(hipStreamSynchronize(r->stream));
+            HIPCHK(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
+            const int rc = issue_update_pass(r, path, batch, positivity);
+            hipGraph_t g = nullptr;
+            const hipError_t ec = hipStreamEndCapture(r->stream, &g);
+            if (rc != MSET_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            HIPCHK(ec);
+            r->graph = g;
+            HIPCHK(hipGraphInstantiate(&r->exec, r->graph, nullptr, nullptr, 0));
+            r->graph_key[0] = path;
+            r->graph_key[1] = batch;
+            r->graph_key[2] = positivity;
+            // the eager pass above was this call's iteration
+        } else {
+            HIPCHK(hipGraphLaunch(r->exec, r->stream));
+        }
+    } else {
+        CHK(issue_update_pass(r, path, batch, positivity));
+    }
+    if (err_out) {
+        HIPCHK(hipStreamSynchronize(r->stream));
+        double s = 0.0;
+        for (mset_ctx* c : r->tilts) {
+            double e = 0.0;
+            CHK(read_errors(c, 0, c->npos, &e));
+            s += e;
+        }
+        *err_out = s / (double)r->tilts.size();
+    }
+    HIPCHK(hipStreamSynchronize(r->stream));
+    return MSET_OK;
+}
+
+int mset_recon_error_pass(mset_recon* r, double* errs) {
+    CHK(check_recon(r));
+    if (!errs) return fail(MSET_EINVAL, "null output");
+    const size_t nv = (size_t)r->n * r->n * r->nz;
+    for (size_t j = 0; j < r->tilts.size(); ++j) {
+        mset_ctx* c = r->tilts[j];
+        HIPCHK(hipMemcpyAsync(c->V.p, r->rec.p, sizeof(float) * nv, hipMemcpyDeviceToDevice, r->stream));
+        CHK(rotate_volume(c, c->V.p, r->vec, r->vec + 3, r->vec + 6, r->angles[j].data(), true));
+        CHK(issue_fwd(c, nullptr));
+        CHK(read_errors(c, 0, c->npos, &errs[j]));
+    }
+    return MSET_OK;
+}
+
+int mset_recon_get(mset_recon* r, float* rec, float* probes, float* xlists, float* ylists) {
+    CHK(check_recon(r));
+    const size_t n2 = (size_t)r->n * r->n;
+    if (rec) HIPCHK(hipMemcpyAsync(rec, r->rec.p, sizeof(float) * n2 * r->nz, hipMemcpyDeviceToHost, r->stream));
+    size_t off = 0;
+    for (size_t j = 0; j < r->tilts.size(); ++j) {
+        mset_ctx* c = r->tilts[j];
+        if (probes)
+            HIPCHK(hipMemcpyAsync(probes + 2 * n2 * j, c->probe.p, sizeof(float2) * n2, hipMemcpyDeviceToHost, r->stream));
+        if (xlists) HIPCHK(hipMemcpyAsync(xlists + off, c->x.p, sizeof(float) * c->npos, hipMemcpyDeviceToHost, r->stream));
+        if (ylists) HIPCHK(hipMemcpyAsync(ylists + off, c->y.p, sizeof(float) * c->npos, hipMemcpyDeviceToHost, r->stream));
+        off += c->npos;
+    }
+    HIPCHK(hipStreamSynchronize(r->stream));
+    return MSET_OK;
+}
+
+int mset_recon_snapshot(mset_recon* r) {
+    CHK(check_recon(r));
+    const size_t n2 = (size_t)r->n * r->n, nv = n2 * r->nz;
+    CHK(r->rec_snap.ensure(nv, "snapshot"));
+    HIPCHK(hipMemcpyAsync(r->rec_snap.p, r->rec.p, sizeof(float) * nv, hipMemcpyDeviceToDevice, r->stream));
+    r->snap_probe.resize(r->tilts.size());
+    r->snap_x.resize(r->tilts.size());
+    r->snap_y.resize(r->tilts.size());
+    for (size_t j = 0; j < r->tilts.size(); ++j) {
+        mset_ctx* c = r->tilts[j];
+        r->snap_probe[j].resize(2 * n2);
+        r->snap_x[j].resize(c->npos);
+        r->snap_y[j].resize(c->npos);
+        HIPCHK(hipMemcpyAsync(r->snap_probe[j].data(), c->probe.p, sizeof(float2) * n2, hipMemcpyDeviceToHost, r->stream));
+        HIPCHK(hipMemcpyAsync(r->snap_x[j].data(), c->x.p, sizeof(float) * c->npos, hipMemcpyDeviceToHost, r->stream));
+        HIPCHK(hipMemcpyAsync(r->snap_y[j].data(), c->y.p, sizeof(float) * c->npos, hipMemcpyDeviceToHost, r->stream));
+    }
+    HIPCHK(hipStreamSynchronize(r->stream));
+    return MSET_OK;
+}
+
+int mset_recon_restore(mset_recon* r) {
+    CHK(check_recon(r));
+    if (!r->rec_snap.p) return fail(MSET_EINVAL, "no snapshot taken");
+    const size_t n2 = (size_t)r->n * r->n, nv = n2 * r->nz;
+    HIPCHK(hipMemcpyAsync(r->rec.p, r->rec_snap.p, sizeof(float) * nv, hipMemcpyDeviceToDevice, r->stream));
+    for (size_t j = 0; j < r->tilts.size(); ++j) {
+        mset_ctx* c = r->tilts[j];
+        HIPCHK(hipMemcpyAsync(c->probe.p, r->snap_probe[j].data(), sizeof(float2) * n2, hipMemcpyHostToDevice, r->stream));
+        HIPCHK(hipMemcpyAsync(c->x.p, r->snap_x[j].data(), sizeof(float) * c->npos, hipMemcpyHostToDevice, r->stream));
+        HIPCHK(hipMemcpyAsync(c->y.p, r->snap_y[j].data(), sizeof(float) * c->npos, hipMemcpyHostToDevice, r->stream));
+    }
+    HIPCHK(hipStreamSynchronize(r->stream));
+    return MSET_OK;
+}
+
+int mset_recon_profile_enable(mset_recon* r, int enable) {
+    if (!r) return fail(MSET_EINVAL, "null reconstruction");
+    r->profiling = enable != 0;
+    if (r->profiling) drop_graph(r);
+    return MSET_OK;
+}
+
+int mset_recon_profile_read(mset_recon* r, const char* name, int* launches, double* total_ms) {
+    if (!r || !name) return fail(MSET_EINVAL, "null argument");
+    HIPCHK(hipSetDevice(r->device));
+    return prof_read(*r->prof, r->stream, name, launches, total_ms);
+}
+
+void* mset_recon_stream(mset_recon* r) { return r ? (void*)r->stream : nullptr; }
+
+}  // extern "C"

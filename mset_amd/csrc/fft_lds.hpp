@@ -1,0 +1,203 @@
+// fft_lds.hpp -- LDS-resident 2D FFT of one N x N complex plane for CDNA4 (gfx950).
+//
+// The plane lives in LDS as float2 s[N*N], row-major [y][x] (x fastest, the MATLAB dim-1 axis).
+// A 2D transform is a pass of 1D transforms along x (rows) followed by a pass along y (columns);
+// each pass is a sequence of Stockham autosort stages with radices Plan<N>::R.  One stage is done by
+// the whole workgroup: every thread loads its butterflies' R points into registers, the workgroup
+// synchronises, every thread applies twiddles and the R-point DFT in registers and writes the R
+// results back in place, and the workgroup synchronises again.  The data is in natural order after
+// every complete 1D pass, so no bit reversal pass exists.
+//
+// Element-wise work is fused into the transform instead of costing LDS sweeps of its own: a `pre`
+// functor is applied to every element as the first row stage loads it (it may also supply the value
+// itself, from global memory, when LOADG is set) and a `post` functor to every element as the last
+// column stage stores it.  Both see the element's natural-order linear index y*N + x, and consecutive
+// threads see consecutive x, so global accesses made inside the functors are coalesced.
+//
+// Forward transforms use exp(-2 pi i / N) (MATLAB fftn), inverse ones exp(+2 pi i / N) without the
+// 1/N^2 factor (callers fold it into a post functor).  Twiddles come from a table tw[k] = W_N^k,
+// k < N, computed on the host in double precision.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace mset {
+
+// ------------------------------------------------------------------ complex helpers
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
+    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// multiply by -i (forward) or +i (inverse)
+template <bool INV> __device__ __forceinline__ float2 mul_mi(float2 a) {
+    return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+// ------------------------------------------------------------------ radix plans
+template <int N> struct Plan;
+template <> struct Plan<16>  { static constexpr int S = 2; static constexpr int R[2] = {4, 4}; };
+template <> struct Plan<24>  { static constexpr int S = 2; static constexpr int R[2] = {8, 3}; };
+template <> struct Plan<32>  { static constexpr int S = 2; static constexpr int R[2] = {8, 4}; };
+template <> struct Plan<48>  { static constexpr int S = 2; static constexpr int R[2] = {16, 3}; };
+template <> struct Plan<64>  { static constexpr int S = 2; static constexpr int R[2] = {8, 8}; };
+template <> struct Plan<96>  { static constexpr int S = 3; static constexpr int R[3] = {8, 4, 3}; };
+template <> struct Plan<128> { static constexpr int S = 2; static constexpr int R[2] = {16, 8}; };
+template <> struct Plan<256> { static constexpr int S = 2; static constexpr int R[2] = {16, 16}; };
+
+// ------------------------------------------------------------------ small DFTs in registers
+// W_16^k = cos(2 pi k/16) - i sin(2 pi k/16) for k = 0..7
+__device__ __forceinline__ float2 w16(int k, bool inv) {
+    constexpr float c[8] = {1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f,
+                            0.0f, -0.38268343236508977f, -0.70710678118654752f, -0.92387953251128674f};
+    constexpr float s[8] = {0.0f, 0.38268343236508977f, 0.70710678118654752f, 0.92387953251128674f,
+                            1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f};
+    return make_float2(c[k], inv ? s[k] : -s[k]);
+}
+
+template <int R, bool INV> struct DFT;
+
+template <bool INV> struct DFT<2, INV> {
+    __device__ __forceinline__ static void run(float2* v) {
+        const float2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    }
+};
+
+template <bool INV> struct DFT<4, INV> {
+    __device__ __forceinline__ static void run(float2* v) {
+        const float2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+        const float2 b0 = cadd(v[1], v[3]), b1 = mul_mi<INV>(csub(v[1], v[3]));
+        v[0] = cadd(a0, b0);
+        v[1] = cadd(a1, b1);
+        v[2] = csub(a0, b0);
+        v[3] = csub(a1, b1);
+    }
+};
+
+// radix-2 split of an R-point DFT into two R/2-point DFTs (R = 8, 16)
+template <int R, bool INV> struct DFTsplit {
+    __device__ __forceinline__ static void run(float2* v) {
+        float2 e[R / 2], o[R / 2];
+#pragma unroll
+        for (int i = 0; i < R / 2; ++i) { e[i] = v[2 * i]; o[i] = v[2 * i + 1]; }
+        DFT<R / 2, INV>::run(e);
+        DFT<R / 2, INV>::run(o);
+#pragma unroll
+        for (int k = 0; k < R / 2; ++k) {
+            float2 t;
+            if (k == 0) t = o[0];
+            else if (2 * k == R / 2) t = mul_mi<INV>(o[k]);
+            else t = cmul(o[k], w16(k * (16 / R), INV));
+            v[k] = cadd(e[k], t);
+            v[k + R / 2] = csub(e[k], t);
+        }
+    }
+};
+template <bool INV> struct DFT<8, INV> : DFTsplit<8, INV> {};
+template <bool INV> struct DFT<16, INV> : DFTsplit<16, INV> {};
+
+template <bool INV> struct DFT<3, INV> {
+    __device__ __forceinline__ static void run(float2* v) {
+        constexpr float s3 = 0.86602540378443865f;    // sin(2 pi / 3)
+        const float2 s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
+        const float2 m = make_float2(v[0].x - 0.5f * s.x, v[0].y - 0.5f * s.y);
+        // forward: -i sin(2pi/3) d ; inverse: +i sin(2pi/3) d
+        const float2 r = INV ? make_float2(-s3 * d.y, s3 * d.x) : make_float2(s3 * d.y, -s3 * d.x);
+        v[0] = cadd(v[0], s);
+        v[1] = cadd(m, r);
+        v[2] = csub(m, r);
+    }
+};
+
+// ------------------------------------------------------------------ functors
+struct NoOp {
+    __device__ __forceinline__ float2 operator()(int, float2 v) const { return v; }
+};
+
+// ------------------------------------------------------------------ one Stockham stage
+// COLS: transform along y (stride N) instead of x.  NS: span (product of the radices already done).
+// FIRST: apply `pre` on load (LOADG: pre supplies the value, LDS is not read).  LAST: apply `post`.
+template <int N, int NT, int R, int NS, bool INV, bool COLS, bool FIRST, bool LOADG, bool LAST, class Pre, class Post>
+__device__ __forceinline__ void fft_stage(float2* __restrict__ s, const float2* __restrict__ tw, const Pre& pre,
+                                          const Post& post) {
+    constexpr int NL = N / R;                 // butterflies per line
+    constexpr int NB = N * NL;                // butterflies in the pass
+    constexpr int CNT = (NB + NT - 1) / NT;   // butterflies per thread
+    constexpr int TWS = N / (NS * R);         // twiddle table stride
+    const int t = threadIdx.x;
+    float2 v[CNT][R];
+#pragma unroll
+    for (int c = 0; c < CNT; ++c) {
+        const int b = t + c * NT;
+        if (NB % NT == 0 || b < NB) {
+            const int line = COLS ? b % N : b / NL;
+            const int j = COLS ? b / N : b % NL;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int e = j + r * NL;
+                const int idx = COLS ? e * N + line : line * N + e;
+                float2 val;
+                if constexpr (FIRST && LOADG) val = pre(idx, make_float2(0.f, 0.f));
+                else if constexpr (FIRST) val = pre(idx, s[idx]);
+                else val = s[idx];
+                v[c][r] = val;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < CNT; ++c) {
+        const int b = t + c * NT;
+        if (NB % NT == 0 || b < NB) {
+            const int line = COLS ? b % N : b / NL;
+            const int j = COLS ? b / N : b % NL;
+            const int k = j % NS;
+            if constexpr (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    float2 w = tw[k * r * TWS];
+                    if (INV) w.y = -w.y;
+                    v[c][r] = cmul(v[c][r], w);
+                }
+            }
+            DFT<R, INV>::run(v[c]);
+            const int base = (j / NS) * NS * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int e = base + r * NS;
+                const int idx = COLS ? e * N + line : line * N + e;
+                float2 val = v[c][r];
+                if constexpr (LAST) val = post(idx, val);
+                s[idx] = val;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+template <int N, int NT, bool INV, bool COLS, int ST, int NS, bool FIRSTPASS, bool LOADG, bool LASTPASS, class Pre,
+          class Post>
+__device__ __forceinline__ void fft_pass(float2* s, const float2* tw, const Pre& pre, const Post& post) {
+    if constexpr (ST < Plan<N>::S) {
+        constexpr int R = Plan<N>::R[ST];
+        fft_stage<N, NT, R, NS, INV, COLS, FIRSTPASS && ST == 0, LOADG, LASTPASS && ST == Plan<N>::S - 1>(s, tw, pre,
+                                                                                                        post);
+        fft_pass<N, NT, INV, COLS, ST + 1, NS * R, FIRSTPASS, LOADG, LASTPASS>(s, tw, pre, post);
+    }
+}
+
+// Full 2D transform of the LDS plane s with fused pre/post element operations.
+template <int N, int NT, bool INV, bool LOADG = false, class Pre = NoOp, class Post = NoOp>
+__device__ __forceinline__ void fft2d(float2* s, const float2* tw, const Pre& pre = Pre(), const Post& post = Post()) {
+    fft_pass<N, NT, INV, false, 0, 1, true, LOADG, false>(s, tw, pre, post);
+    fft_pass<N, NT, INV, true, 0, 1, false, false, true>(s, tw, pre, post);
+}
+
+}  // namespace mset

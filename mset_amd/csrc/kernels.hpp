@@ -1,0 +1,57 @@
+// kernels.hpp -- declarations shared by the kernels and the host engine.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace mset {
+
+enum { GRAD_REF = 0, GRAD_EXACT = 1 };
+enum { WANT_PROBE = 1, WANT_POS = 2 };
+
+// Arguments of the per-position kernels (passed by value).  Plane data is [y][x], N x N.
+struct PosArgs {
+    int nz, L, bin;              // volume depth, number of planes ceil(nz/bin), slice binning
+    float sigma, dx;             // interaction constant, voxel size: phase = sigma * (dx * sum V)
+    float cent;                  // round((N+1)/2), the 1-based centre of the scan coordinates
+    float s1, s2, s3;            // step sizes (object, probe, position)
+    int grad_mode;               // GRAD_REF or GRAD_EXACT
+    int want;                    // WANT_PROBE | WANT_POS
+    const float2* tw;            // W_N^k, k < N
+    const float2* prop;          // propagator, unshifted
+    const float* sqrtM;          // sqrt of the measured patterns, unshifted, [k][y][x]
+    float* V;                    // K_PLAIN: volume [z][y][x], updated in place
+    const float2* trans;         // K_BATCH / K_FWD: transmission planes [a][y][x]
+    float2* probe;               // K_PLAIN: probe, updated in place
+    const float2* probeF;        // K_BATCH / K_FWD: fftn(probe)
+    float* xlist;                // scan positions (MATLAB 1-based pixels); K_PLAIN updates them
+    float* ylist;
+    const int* order;            // optional list of position indices
+    int first, count;            // K_BATCH / K_FWD: positions order[first .. first+count)
+    double* err_pos;             // per position: sum over pixels of (sqrt M - sqrt I)^2
+    float* inten;                // K_FWD, optional: |Psi|^2 per position, DC-centred (MATLAB layout)
+    float2* save;                // per-slot scratch: (L + 2) planes
+    float* gslab;                // K_BATCH: per-slot object-gradient slab, L planes
+    float2* pslab;               // K_BATCH: per-slot probe-gradient plane (Fourier domain)
+    float* dpos;                 // K_BATCH: per-position update terms (2 per position)
+};
+
+bool plane_size_supported(int n);
+int position_threads(int n);
+hipError_t launch_plain(int n, const PosArgs& A, int k, hipStream_t st);
+hipError_t launch_batch(int n, const PosArgs& A, int grid, hipStream_t st);
+hipError_t launch_fwd(int n, const PosArgs& A, int grid, hipStream_t st);
+hipError_t launch_plane_fft(int n, const float2* in, float2* out, const float2* tw, int dir, float alpha, float beta,
+                            hipStream_t st);
+hipError_t occupancy_batch(int n, int* blocks_per_cu);
+
+// element-wise kernels (elementwise.hip)
+hipError_t launch_ingest_patterns(const float* M, float* sqrtM, int n, long long npos, hipStream_t st);
+hipError_t launch_transmission(const float* V, float2* trans, int n, int nz, int bin, float sigma, float dx,
+                               hipStream_t st);
+hipError_t launch_fill(void* p, size_t bytes, hipStream_t st);
+hipError_t launch_reduce_slabs(const float* slab, int nslab, size_t n, float* out, hipStream_t st);
+hipError_t launch_reduce_cslabs(const float2* slab, int nslab, size_t n, float2* out, hipStream_t st);
+hipError_t launch_update_volume(float* V, const float* G, int n, int nz, int bin, float s1, hipStream_t st);
+hipError_t launch_update_positions(float* x, float* y, const float* dpos, int npos, float s3, hipStream_t st);
+hipError_t launch_positivity(float* V, size_t n, hipStream_t st);
+
+}  // namespace mset

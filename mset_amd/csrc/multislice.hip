@@ -1,0 +1,406 @@
+// multislice.hip -- fused multislice forward / adjoint kernels (CDNA4, gfx950).
+//
+// One workgroup owns one probe position at a time and keeps the N x N wave in LDS for the whole
+// forward and adjoint sweep.  Everything the reference does per position with separate launches
+// (src/CUDA/mset.cu:107-333: probe shift, per-plane x t / FFT / x p / IFFT, exit FFT, residual,
+// per-plane adjoint and gradient, probe and position terms) is one pass through this routine; the
+// element-wise steps ride on the FFT stages' loads and stores (fft_lds.hpp).
+//
+// Three kinds of launch share it:
+//   K_PLAIN  one position per launch, sequential semantics of src/Run_MSET_recon.m:134-158 and
+//            src/CUDA/mset.cu:107-333: the transmission is recomputed from V, and V, the probe and
+//            the position are updated in place before the next launch.
+//   K_BATCH  many positions per launch at a fixed (V, probe): a persistent grid of workgroups, each
+//            accumulating its positions' object gradient into a private slab (L planes) and its
+//            probe-gradient term into a private Fourier-domain plane; slabs are summed afterwards in
+//            a fixed order, so results are bitwise repeatable and no atomics are used.
+//   K_FWD    forward + loss only (run_error_mset, src/CUDA/mset.cu:557-696), many positions.
+//
+// Conventions: Fourier data is unshifted (DC at [0][0]); the propagator and the measured patterns are
+// ifftshifted once at upload (src/CUDA/mex_main.cu:80, mset.cu:208).  Real-space data keeps the
+// MATLAB layout; the multislice operator commutes with circular shifts, so this equals MATLAB's
+// fftshift(fftn(ifftshift(.))) chain (STEP02_FORWARD_4D.m:32-36).  The sub-pixel shift follows
+// Func_generate_probe_wave_scan_sub_pixel.m:4-27 (grid -n/2..n/2-1, so the Nyquist bin uses -n/2).
+#include <hip/hip_runtime.h>
+#include "fft_lds.hpp"
+#include "kernels.hpp"
+
+namespace mset {
+
+template <int N> struct Cfg;
+template <> struct Cfg<16>  { static constexpr int NT = 64; };
+template <> struct Cfg<24>  { static constexpr int NT = 64; };
+template <> struct Cfg<32>  { static constexpr int NT = 64; };
+template <> struct Cfg<48>  { static constexpr int NT = 192; };
+template <> struct Cfg<64>  { static constexpr int NT = 256; };
+template <> struct Cfg<96>  { static constexpr int NT = 576; };
+template <> struct Cfg<128> { static constexpr int NT = 1024; };
+
+enum { K_PLAIN = 0, K_BATCH = 1, K_FWD = 2 };
+
+// Sum of one double per thread over the workgroup; every thread gets the total.
+template <int NT> __device__ __forceinline__ double block_sum(double v, double* red) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < (NT + 63) / 64; ++i) s += red[i];
+    return s;
+}
+
+// grid value of the sub-pixel shift ramp at unshifted frequency index j (Nyquist -> -N/2)
+template <int N> __device__ __forceinline__ double shift_grid(int j) {
+    return (double)((j + N / 2) % N) - (double)(N / 2);
+}
+
+// transmission of plane a at pixel i from the volume: exp(1i*sigma*(sum_{bin a} V)*dx)
+// (STEP02_FORWARD_4D.m:12-20; same rounding order: sum, x dx, x sigma)
+__device__ __forceinline__ float2 trans_from_volume(const float* __restrict__ V, int n2, int z0, int z1, int i,
+                                                    float dx, float sigma) {
+    float s = 0.f;
+    for (int z = z0; z < z1; ++z) s += V[(size_t)z * n2 + i];
+    const float ph = sigma * (s * dx);
+    float sn, cs;
+    sincosf(ph, &sn, &cs);
+    return make_float2(cs, sn);
+}
+
+template <int N, int NT, int KIND>
+__device__ void process_position(const PosArgs& A, int k, int slot, float2* s, float2* tw, float2* phx,
+                                 float2* phy, double* red) {
+    constexpr int N2 = N * N;
+    const int t = threadIdx.x;
+    const int L = A.L;
+    const float inv_n2 = 1.0f / (float)N2;
+    float2* save = A.save + (size_t)slot * (L + 2) * N2;   // slots 0..L waves, L+1 Fourier incident wave
+    const bool want_pos = (A.want & WANT_POS) != 0;
+    const bool want_probe = (A.want & WANT_PROBE) != 0;
+    const bool exact = A.grad_mode == GRAD_EXACT;
+
+    // ---- sub-pixel shift factors, separable: Phi(jy, jx) = phy[jy] * phx[jx]  (double-precision phase)
+    const double row = (double)A.xlist[k], col = (double)A.ylist[k];
+    const double dxs = row - (double)A.cent, dys = col - (double)A.cent;
+    for (int j = t; j < 2 * N; j += NT) {
+        const int jj = j < N ? j : j - N;
+        const double d = j < N ? dxs : dys;
+        const double a = -2.0 * M_PI * d * shift_grid<N>(jj) / (double)N;
+        double sn, cs;
+        sincos(a, &sn, &cs);
+        (j < N ? phx : phy)[jj] = make_float2((float)cs, (float)sn);
+    }
+    __syncthreads();
+    auto phase = [&](int idx) { return cmul(phy[idx / N], phx[idx % N]); };
+
+    // ---- incident wave psi0 = ifftn(fftn(probe) .* Phi)
+    if constexpr (KIND == K_PLAIN) {
+        const float2* probe = A.probe;
+        fft2d<N, NT, false, true>(s, tw, [&](int idx, float2) { return probe[idx]; },
+                                  [&](int idx, float2 v) {
+                                      const float2 f = cmul(v, phase(idx));
+                                      save[(size_t)(L + 1) * N2 + idx] = f;
+                                      return f;
+                                  });
+        fft2d<N, NT, true>(s, tw, NoOp(), [&](int idx, float2 v) {
+            const float2 w = cscale(v, inv_n2);
+            save[idx] = w;
+            return w;
+        });
+    } else {
+        const float2* pf = A.probeF;
+        fft2d<N, NT, true, true>(s, tw,
+                                 [&](int idx, float2) {
+                                     const float2 f = cmul(pf[idx], phase(idx));
+                                     if (KIND == K_BATCH) save[(size_t)(L + 1) * N2 + idx] = f;
+                                     return f;
+                                 },
+                                 [&](int idx, float2 v) {
+                                     const float2 w = cscale(v, inv_n2);
+                                     if (KIND == K_BATCH) save[idx] = w;
+                                     return w;
+                                 });
+    }
+
+    // ---- forward sweep: psi_a = ifftn(fftn(psi_{a-1} .* t_a) .* p)
+    const float2* prop = A.prop;
+    for (int a = 0; a < L; ++a) {
+        const int z0 = a * A.bin, z1 = min((a + 1) * A.bin, A.nz);
+        if constexpr (KIND == K_PLAIN) {
+            const float* V = A.V;
+            fft2d<N, NT, false>(s, tw,
+                                [&](int idx, float2 v) { return cmul(v, trans_from_volume(V, N2, z0, z1, idx, A.dx, A.sigma)); },
+                                [&](int idx, float2 v) { return cmul(v, prop[idx]); });
+        } else {
+            const float2* ta = A.trans + (size_t)a * N2;
+            fft2d<N, NT, false>(s, tw, [&](int idx, float2 v) { return cmul(v, ta[idx]); },
+                                [&](int idx, float2 v) { return cmul(v, prop[idx]); });
+        }
+        float2* sv = save + (size_t)(a + 1) * N2;
+        fft2d<N, NT, true>(s, tw, NoOp(), [&](int idx, float2 v) {
+            const float2 w = cscale(v, inv_n2);
+            if (KIND != K_FWD) sv[idx] = w;
+            return w;
+        });
+    }
+
+    // ---- exit wave, intensity, amplitude residual (STEP02_FORWARD_4D.m:36-49), r .* conj(p) fused
+    double err = 0.0;
+    {
+        const float* M = A.sqrtM + (size_t)k * N2;
+        float* Iout = A.inten ? A.inten + (size_t)k * N2 : nullptr;
+        fft2d<N, NT, false>(s, tw, NoOp(), [&](int idx, float2 v) {
+            const float I = v.x * v.x + v.y * v.y;
+            if (KIND == K_FWD && Iout) Iout[((idx / N + N / 2) % N) * N + (idx % N + N / 2) % N] = I;
+            const float m = M[idx];
+            const float d = m - sqrtf(I);
+            err += (double)(d * d);
+            if (KIND == K_FWD) return v;
+            const float f = 1.0f - m / sqrtf(I + 1e-30f);
+            return cmulc(make_float2(v.x * f, v.y * f), prop[idx]);
+        });
+    }
+    err = block_sum<NT>(err, red);
+    if (t == 0) A.err_pos[k] = err;
+    if constexpr (KIND == K_FWD) return;
+
+    // ---- adjoint sweep (STEP03_BACK.m:12-24, mset.cu:232-264)
+    for (int a = L - 1; a >= 0; --a) {
+        const int z0 = a * A.bin, z1 = min((a + 1) * A.bin, A.nz);
+        const float2* ws = save + (size_t)(exact ? a : a + 1) * N2;
+        if constexpr (KIND == K_PLAIN) {
+            float* V = A.V;
+            const float s1 = A.s1;
+            fft2d<N, NT, true>(s, tw, NoOp(), [&](int idx, float2 v) {
+                const float2 b = cscale(v, inv_n2);
+                const float2 tc = cconj(trans_from_volume(V, N2, z0, z1, idx, A.dx, A.sigma));
+                const float2 q = cmul(tc, cmulc(b, ws[idx]));
+                const float gr = -q.y;                       // Re(1i * conj(t) * conj(psi) * b)
+                for (int z = z0; z < z1; ++z) V[(size_t)z * N2 + idx] += s1 * gr;
+                return cmul(b, tc);
+            });
+        } else {
+            const float2* ta = A.trans + (size_t)a * N2;
+            float* g = A.gslab + ((size_t)slot * L + a) * N2;
+            fft2d<N, NT, true>(s, tw, NoOp(), [&](int idx, float2 v) {
+                const float2 b = cscale(v, inv_n2);
+                const float2 tc = cconj(ta[idx]);
+                const float2 q = cmul(tc, cmulc(b, ws[idx]));
+                g[idx] += -q.y;
+                return cmul(b, tc);
+            });
+        }
+        if (a > 0) fft2d<N, NT, false>(s, tw, NoOp(), [&](int idx, float2 v) { return cmulc(v, prop[idx]); });
+        else fft2d<N, NT, false>(s, tw);
+    }
+    // s now holds fftn(back_wave_f) (natural-order Fourier domain)
+
+    // ---- probe term in the Fourier domain: shift(bw, -d) = ifftn(fftn(bw) .* conj(Phi))
+    if constexpr (KIND == K_BATCH) {
+        if (want_probe) {
+            float2* ps = A.pslab + (size_t)slot * N2;
+            for (int idx = t; idx < N2; idx += NT) ps[idx] = cadd(ps[idx], cmulc(s[idx], phase(idx)));
+        }
+    }
+
+    // ---- position terms (STEP03_BACK.m:89-92)
+    double dr = 0.0, dc = 0.0;
+    bool have_bw = false;
+    if (want_pos) {
+        if (exact) {
+            // Parseval: sum_x bw conj(-dpsi0/drow) = (1/N^2) sum_q B(q) conj(Psi0(q)) (-2 pi i g/N)
+            const float2* F0 = save + (size_t)(L + 1) * N2;
+            for (int idx = t; idx < N2; idx += NT) {
+                const float2 c = cmulc(s[idx], F0[idx]);       // B conj(Psi0)
+                const double gx = shift_grid<N>(idx % N), gy = shift_grid<N>(idx / N);
+                // Re(c * (-i) * 2 pi g / N) = Im(c) * 2 pi g / N
+                dr += (double)c.y * gx;
+                dc += (double)c.y * gy;
+            }
+            const double f = 2.0 * M_PI / (double)N / (double)N2;
+            dr = block_sum<NT>(dr, red) * f;
+            dc = block_sum<NT>(dc, red) * f;
+        } else {
+            // back_wave_f = ifftn(B); MATLAB gradient() of conj(psi0), one-sided at the edges
+            fft2d<N, NT, true>(s, tw, NoOp(), [&](int, float2 v) { return cscale(v, inv_n2); });
+            have_bw = true;
+            const float2* p0 = save;
+            for (int idx = t; idx < N2; idx += NT) {
+                const int x = idx % N, y = idx / N;
+                const float2 bw = s[idx];
+                float2 gx, gy;
+                if (x == 0) gx = csub(p0[idx + 1], p0[idx]);
+                else if (x == N - 1) gx = csub(p0[idx], p0[idx - 1]);
+                else gx = cscale(csub(p0[idx + 1], p0[idx - 1]), 0.5f);
+                if (y == 0) gy = csub(p0[idx + N], p0[idx]);
+                else if (y == N - 1) gy = csub(p0[idx], p0[idx - N]);
+                else gy = cscale(csub(p0[idx + N], p0[idx - N]), 0.5f);
+                // Re(conj(g) * bw)
+                dr += (double)(gx.x * bw.x + gx.y * bw.y);
+                dc += (double)(gy.x * bw.x + gy.y * bw.y);
+            }
+            dr = block_sum<NT>(dr, red);
+            dc = block_sum<NT>(dc, red);
+        }
+    }
+
+    if constexpr (KIND == K_BATCH) {
+        if (t == 0 && want_pos) {
+            A.dpos[2 * k] = (float)dr;
+            A.dpos[2 * k + 1] = (float)dc;
+        }
+    } else {
+        // ---- sequential updates (STEP03_BACK.m:84-92, Run_MSET_recon.m:155-157)
+        if (want_probe) {
+            float2* probe = A.probe;
+            const float s2 = A.s2;
+            if (have_bw) {
+                // MATLAB shifts back_wave_f: fftn(bw) .* conj(Phi) -> ifftn
+                fft2d<N, NT, false>(s, tw, NoOp(), [&](int idx, float2 v) { return cmulc(v, phase(idx)); });
+                fft2d<N, NT, true>(s, tw, NoOp(), [&](int idx, float2 v) {
+                    const float2 w = cscale(v, inv_n2);
+                    probe[idx] = csub(probe[idx], cscale(w, s2));
+                    return w;
+                });
+            } else {
+                fft2d<N, NT, true>(s, tw, [&](int idx, float2 v) { return cmulc(v, phase(idx)); },
+                                   [&](int idx, float2 v) {
+                                       const float2 w = cscale(v, inv_n2);
+                                       probe[idx] = csub(probe[idx], cscale(w, s2));
+                                       return w;
+                                   });
+            }
+        }
+        if (t == 0 && want_pos) {
+            A.xlist[k] = (float)(row + (double)A.s3 * dr);
+            A.ylist[k] = (float)(col + (double)A.s3 * dc);
+        }
+    }
+}
+
+template <int N, int NT> __device__ __forceinline__ void load_twiddles(float2* tw, const float2* g) {
+    for (int i = threadIdx.x; i < N; i += NT) tw[i] = g[i];
+    __syncthreads();
+}
+
+template <int N>
+__global__ __launch_bounds__(Cfg<N>::NT) void k_plain(PosArgs A, int k) {
+    constexpr int NT = Cfg<N>::NT;
+    __shared__ float2 s[N * N];
+    __shared__ float2 tw[N], phx[N], phy[N];
+    __shared__ double red[(NT + 63) / 64];
+    load_twiddles<N, NT>(tw, A.tw);
+    process_position<N, NT, K_PLAIN>(A, k, 0, s, tw, phx, phy, red);
+}
+
+template <int N>
+__global__ __launch_bounds__(Cfg<N>::NT) void k_batch(PosArgs A) {
+    constexpr int NT = Cfg<N>::NT;
+    __shared__ float2 s[N * N];
+    __shared__ float2 tw[N], phx[N], phy[N];
+    __shared__ double red[(NT + 63) / 64];
+    load_twiddles<N, NT>(tw, A.tw);
+    for (int p = blockIdx.x; p < A.count; p += gridDim.x) {
+        const int k = A.order ? A.order[A.first + p] : A.first + p;
+        process_position<N, NT, K_BATCH>(A, k, blockIdx.x, s, tw, phx, phy, red);
+        __syncthreads();
+    }
+}
+
+template <int N>
+__global__ __launch_bounds__(Cfg<N>::NT) void k_fwd(PosArgs A) {
+    constexpr int NT = Cfg<N>::NT;
+    __shared__ float2 s[N * N];
+    __shared__ float2 tw[N], phx[N], phy[N];
+    __shared__ double red[(NT + 63) / 64];
+    load_twiddles<N, NT>(tw, A.tw);
+    for (int p = blockIdx.x; p < A.count; p += gridDim.x) {
+        const int k = A.order ? A.order[A.first + p] : A.first + p;
+        process_position<N, NT, K_FWD>(A, k, blockIdx.x, s, tw, phx, phy, red);
+        __syncthreads();
+    }
+}
+
+// fftn (dir = 0) or ifftn with 1/N^2 (dir = 1) of one plane, optionally scaled and added:
+// out = beta * out + alpha * T(in)   (beta = 0: plain store)
+template <int N>
+__global__ __launch_bounds__(Cfg<N>::NT) void k_plane_fft(const float2* in, float2* out, const float2* twg, int dir,
+                                                         float alpha, float beta) {
+    constexpr int NT = Cfg<N>::NT;
+    __shared__ float2 s[N * N];
+    __shared__ float2 tw[N];
+    load_twiddles<N, NT>(tw, twg);
+    const float sc = dir ? alpha / (float)(N * N) : alpha;
+    auto ld = [&](int idx, float2) { return in[idx]; };
+    auto st = [&](int idx, float2 v) {
+        const float2 w = cscale(v, sc);
+        out[idx] = beta == 0.f ? w : cadd(cscale(out[idx], beta), w);
+        return v;
+    };
+    if (dir) fft2d<N, NT, true, true>(s, tw, ld, st);
+    else fft2d<N, NT, false, true>(s, tw, ld, st);
+}
+
+// ------------------------------------------------------------------ host launchers
+template <int N> static int nt_of() { return Cfg<N>::NT; }
+
+#define MSET_DISPATCH(N, BODY)                                              \
+    switch (N) {                                                            \
+        case 16: { constexpr int NN = 16; BODY; } break;                    \
+        case 24: { constexpr int NN = 24; BODY; } break;                    \
+        case 32: { constexpr int NN = 32; BODY; } break;                    \
+        case 48: { constexpr int NN = 48; BODY; } break;                    \
+        case 64: { constexpr int NN = 64; BODY; } break;                    \
+        case 96: { constexpr int NN = 96; BODY; } break;                    \
+        case 128: { constexpr int NN = 128; BODY; } break;                  \
+        default: return hipErrorInvalidValue;                               \
+    }
+
+bool plane_size_supported(int n) {
+    return n == 16 || n == 24 || n == 32 || n == 48 || n == 64 || n == 96 || n == 128;
+}
+
+int position_threads(int n) {
+    int r = 0;
+    switch (n) {
+        case 16: r = Cfg<16>::NT; break;
+        case 24: r = Cfg<24>::NT; break;
+        case 32: r = Cfg<32>::NT; break;
+        case 48: r = Cfg<48>::NT; break;
+        case 64: r = Cfg<64>::NT; break;
+        case 96: r = Cfg<96>::NT; break;
+        case 128: r = Cfg<128>::NT; break;
+        default: r = 0;
+    }
+    return r;
+}
+
+hipError_t launch_plain(int n, const PosArgs& A, int k, hipStream_t st) {
+    MSET_DISPATCH(n, hipLaunchKernelGGL(k_plain<NN>, dim3(1), dim3(Cfg<NN>::NT), 0, st, A, k));
+    return hipGetLastError();
+}
+
+hipError_t launch_batch(int n, const PosArgs& A, int grid, hipStream_t st) {
+    MSET_DISPATCH(n, hipLaunchKernelGGL(k_batch<NN>, dim3(grid), dim3(Cfg<NN>::NT), 0, st, A));
+    return hipGetLastError();
+}
+
+hipError_t launch_fwd(int n, const PosArgs& A, int grid, hipStream_t st) {
+    MSET_DISPATCH(n, hipLaunchKernelGGL(k_fwd<NN>, dim3(grid), dim3(Cfg<NN>::NT), 0, st, A));
+    return hipGetLastError();
+}
+
+hipError_t launch_plane_fft(int n, const float2* in, float2* out, const float2* tw, int dir, float alpha, float beta,
+                            hipStream_t st) {
+    MSET_DISPATCH(n, hipLaunchKernelGGL(k_plane_fft<NN>, dim3(1), dim3(Cfg<NN>::NT), 0, st, in, out, tw, dir, alpha, beta));
+    return hipGetLastError();
+}
+
+hipError_t occupancy_batch(int n, int* blocks_per_cu) {
+    MSET_DISPATCH(n, return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batch<NN>, Cfg<NN>::NT, 0));
+    return hipSuccess;
+}
+
+}  // namespace mset

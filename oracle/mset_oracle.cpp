@@ -150,7 +150,7 @@ template <typename T> void fft_axis(cx<T>* a, int n0, int n1, int n2, int axis, 
     const int lines = n0 * n1 * n2 / n;
     get_plan<T>(n);
     // small transforms run on the calling thread: thread start-up would dominate them
-#pragma omp parallel if ((long long)lines * n >= 32768)
+#pragma omp parallel if ((long long)lines * n >= 4096)
     {
         std::vector<cx<T>> bi(n), bo(n);
 #pragma omp for schedule(static)

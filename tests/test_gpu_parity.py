@@ -1,0 +1,193 @@
+"""GPU parity: the HIP path through the C ABI against the FP64 oracle and the reference's golden vectors.
+
+Tolerances (FP32 device arithmetic against FP64): 1e-4 relative on exit waves and losses, 1e-3 on
+gradients and on the updates they produce; rotation 1e-5.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.cases import small_case
+from tests.conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+WAVE_TOL = 1e-4
+LOSS_TOL = 1e-4
+GRAD_TOL = 1e-3
+
+
+@pytest.fixture(scope='module')
+def mset():
+    from mset_amd import lib
+    ctx = lib.Context(0)
+    yield ctx
+    ctx.close()
+
+
+def _phys(man):
+    E0 = man['E0_kV'] * 1e3
+    return O.wavelength(E0), O.interaction_sigma(E0), man['potential_pixelsize']
+
+
+def assert_update_close(new, onew, old, nupd, tol=GRAD_TOL):
+    """new - old must match onew - old to `tol` relative, above the floor of FP32 storage rounding."""
+    d, od = np.asarray(new) - old, np.asarray(onew) - old
+    floor = nupd * np.finfo(np.float32).eps * np.linalg.norm(np.ravel(old))
+    err = np.linalg.norm(np.ravel(d - od))
+    assert err <= tol * np.linalg.norm(np.ravel(od)) + floor, (err, np.linalg.norm(np.ravel(od)), floor)
+    assert rel(new, onew) < 1e-5
+
+
+def _upload_case(ctx, c, steps=(0.0, 0.0, 0.0), mode=0, V=None):
+    ctx.upload(c['V'] if V is None else V, c['probe'], c['prop'], c['xl'], c['yl'], c['M'], c['sig'], c['dx'],
+               slice_binning=c['bin'], step_size=steps, n_scan=(c['npos'], 1), grad_mode=mode)
+
+
+# ------------------------------------------------------------------ reference golden vectors
+def test_waves_match_saved_multislice_fixture(mset, cuau):
+    """STEP02_FORWARD_4D save_wave_f of position 576, tilt 8 (MSET_recon.mat) and the oracle."""
+    man, s = cuau['man'], cuau['snap']
+    lam, sig, dx = _phys(man)
+    j = man['Nth_angle'] - 1
+    print('case: CuAu N=24 tilt', j + 1, 'position', man['k'], flush=True)
+    mset.upload(s['RVol'], s['probe_wfn'][j], s['prop'], s['scan_pos'][j, 0], s['scan_pos'][j, 1], cuau['meas'][j],
+                sig, dx, slice_binning=1, n_scan=(24, 24))
+    w = mset.waves(man['k'] - 1)
+    assert rel(w[0], s['tmp_probe_wfn']) < 1e-5
+    for a in range(24):
+        assert rel(w[a + 1], s['save_wave_f'][a]) < 2e-5, a
+    save, _, _, _ = O.forward(s['RVol'], O.subpixel_shift(s['probe_wfn'][j], man['row'], man['col']), s['prop'],
+                              cuau['meas'][j][man['k'] - 1], 1, sig, dx)
+    for a in range(24):
+        assert rel(w[a + 1], save[a]) < WAVE_TOL, a
+
+
+def test_error_pass_matches_saved_per_tilt_errors(mset, cuau):
+    """Fixture 7: error pass from the saved rec (device rotation + device forward/loss), all 8 tilts."""
+    man, s = cuau['man'], cuau['snap']
+    lam, sig, dx = _phys(man)
+    errs = []
+    for j in range(8):
+        print('case: CuAu error pass tilt', j + 1, flush=True)
+        mset.upload(s['rec'], s['probe_wfn'][j], s['prop'], s['scan_pos'][j, 0], s['scan_pos'][j, 1],
+                    cuau['meas'][j], sig, dx, slice_binning=1, n_scan=(24, 24))
+        mset.rotate_resident(s['tilt_angles'][j], inverse=True)
+        errs.append(mset.error_resident())
+    errs = np.array(errs)
+    assert np.abs(errs / s['error'] - 1).max() < 5e-5, errs / s['error'] - 1
+
+
+def test_rotation_matches_saved_rotated_volume(mset, cuau):
+    """Fixture 6: RVol = Func_inv_rot_3Dvol_FourierShear(rec, tilt 8)."""
+    s = cuau['snap']
+    R, _ = O.rot_matrix(s['tilt_angles'][7], inverse=True)
+    out = mset.rotate(s['rec'], R)
+    assert rel(out, s['RVol']) < 5e-6
+
+
+@pytest.mark.parametrize('shape,ang', [((24, 32, 32), (0.0, 37.0, 0.0)), ((64, 64, 64), (12.0, -50.0, 7.0)),
+                                       ((48, 96, 96), (-30.0, 0.0, 0.0)), ((32, 128, 128), (0.0, 0.0, 22.0)),
+                                       ((96, 48, 64), (5.0, 60.0, -15.0))])
+def test_rotation_vs_oracle(mset, shape, ang):
+    rng = np.random.default_rng(sum(shape))
+    v = rng.standard_normal(shape).astype(np.float32)
+    R, _ = O.rot_matrix(ang)
+    print('case: rotation', shape, ang, flush=True)
+    assert rel(mset.rotate(v, R), O.rotate(v, R)) < 1e-5
+
+
+# ------------------------------------------------------------------ operator parity on synthetic states
+@pytest.mark.parametrize('n,nz,binning,mode', [(24, 6, 2, O.MODE_REF), (32, 12, 3, O.MODE_EXACT),
+                                               (64, 8, 4, O.MODE_REF), (128, 16, 4, O.MODE_EXACT),
+                                               (48, 5, 2, O.MODE_EXACT), (96, 4, 1, O.MODE_REF)])
+def test_grad_at_fixed_state_vs_oracle(mset, n, nz, binning, mode):
+    c = small_case(n=n, nz=nz, binning=binning, npos=5, seed=n + nz, spread=n / 8)
+    print('case: grad', n, nz, binning, mode, flush=True)
+    _upload_case(mset, c, mode=mode)
+    dV, dp, dpos, err = mset.grad()
+    odV, odp, odpos, oep = O.grad_fixed(c['V'], c['probe'], c['prop'], c['bprop'], c['xl'], c['yl'], c['M'],
+                                        c['bin'], c['sig'], c['dx'], mode=mode)
+    oerr = oep.sum() / c['npos']          # oracle terms are per-position means
+    assert abs(err / oerr - 1) < LOSS_TOL
+    assert rel(dV, odV) < GRAD_TOL
+    assert rel(dp, odp) < GRAD_TOL
+    assert rel(dpos, odpos) < GRAD_TOL
+    w = mset.waves(2)
+    psi0 = O.subpixel_shift(c['probe'], c['xl'][2], c['yl'][2])
+    save, _, _, _ = O.forward(c['V'], psi0, c['prop'], c['M'][2], c['bin'], c['sig'], c['dx'])
+    assert rel(w[0], psi0) < WAVE_TOL
+    assert rel(w[-1], save[-1]) < WAVE_TOL
+
+
+@pytest.mark.parametrize('n,nz,binning,mode', [(24, 6, 2, O.MODE_REF), (32, 9, 3, O.MODE_EXACT),
+                                               (64, 8, 4, O.MODE_REF)])
+def test_sequential_run_vs_oracle(mset, n, nz, binning, mode):
+    """'run_mset' semantics: per-position updates of V, probe and position (Run_MSET_recon.m:134-158)."""
+    c = small_case(n=n, nz=nz, binning=binning, npos=6, seed=3 * n + nz, spread=n / 8)
+    steps = (30.0, 2e-3, 2e-2)
+    print('case: run', n, nz, binning, mode, flush=True)
+    _upload_case(mset, c, steps=steps, mode=mode)
+    V, pr, xl, yl, err = mset.run()
+    oV, opr, oxl, oyl, oerr, _ = O.run_tilt(c['V'], c['probe'], c['prop'], c['bprop'], c['xl'], c['yl'], c['M'],
+                                            c['npos'], 1, c['bin'], c['sig'], c['dx'], steps, mode=mode)
+    assert abs(err / oerr - 1) < LOSS_TOL
+    # the state is stored in single precision (as the reference stores RVol and the probe): after npos
+    # updates each element carries up to ~npos ulp of rounding on top of the gradient tolerance
+    assert_update_close(V, oV, c['V'], c['npos'])
+    assert_update_close(pr, opr, c['probe'], c['npos'])
+    d = np.c_[xl - c['xl'], yl - c['yl']]
+    od = np.c_[oxl - c['xl'], oyl - c['yl']]
+    assert rel(d, od) < 5e-3     # positions are stored in single precision (MATLAB scan_pos is single)
+
+
+def test_error_pass_vs_oracle_and_zero_step_paths(mset):
+    c = small_case(n=32, nz=8, binning=2, npos=7, seed=11)
+    _upload_case(mset, c)
+    e_fwd = mset.error_resident()
+    oerr, _ = O.error_tilt(c['V'], c['probe'], c['prop'], c['xl'], c['yl'], c['M'], c['npos'], 1, c['bin'],
+                           c['sig'], c['dx'])
+    assert abs(e_fwd / oerr - 1) < LOSS_TOL
+    e_run = mset.run_resident()            # steps are zero: the state must not move
+    V, pr, xl, yl = mset.download()
+    assert np.array_equal(V, c['V'].astype(np.float32)) and np.array_equal(xl, c['xl'])
+    assert abs(e_run / e_fwd - 1) < 1e-6
+    e_b = mset.run_batched(3)
+    assert abs(e_b / e_fwd - 1) < 1e-6
+
+
+def test_batched_grad_is_bitwise_repeatable_and_additive(mset):
+    c = small_case(n=64, nz=8, binning=2, npos=9, seed=5)
+    _upload_case(mset, c)
+    g1 = mset.grad()
+    g2 = mset.grad()
+    assert np.array_equal(g1[0], g2[0]) and np.array_equal(g1[1], g2[1])
+    a = mset.grad(0, 4)
+    b = mset.grad(4, 5)
+    assert rel(a[0] + b[0], g1[0]) < 1e-5
+    assert rel(np.r_[a[2], b[2]], g1[2]) < 1e-6
+
+
+def test_mini_batch_of_one_equals_sequential(mset):
+    c = small_case(n=32, nz=6, binning=2, npos=5, seed=8)
+    steps = (30.0, 2e-3, 2e-2)
+    _upload_case(mset, c, steps=steps)
+    V1, p1, x1, y1, e1 = mset.run()
+    _upload_case(mset, c, steps=steps)
+    e2 = mset.run_batched(1)
+    V2, p2, x2, y2 = mset.download()
+    assert rel(V2 - c['V'], V1 - c['V']) < 1e-4
+    assert rel(p2 - c['probe'], p1 - c['probe']) < 1e-4
+    assert abs(e2 / e1 - 1) < 1e-5
+
+
+def test_unsupported_method_and_bad_shapes_raise(mset):
+    from mset_amd import lib
+    c = small_case(n=32, nz=4, binning=1, npos=2, seed=1)
+    _upload_case(mset, c)
+    with pytest.raises(lib.MsetError) as e:
+        mset.run(method=lib.METHOD_SSET)
+    assert e.value.code == lib.MSET_EUNSUPPORTED
+    with pytest.raises(lib.MsetError):
+        mset.upload(np.zeros((4, 32, 30), np.float32), c['probe'], c['prop'], c['xl'], c['yl'],
+                    np.zeros((2, 32, 30), np.float32), c['sig'], c['dx'])
