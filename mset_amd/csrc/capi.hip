@@ -411,13 +411,14 @@ int upload_impl(mset_ctx* c, const mset_params* p, const float* RVol, const floa
     CHK(c->err.ensure(c->npos, "errors"));
     CHK(c->Mtmp.ensure(n2 * c->npos, "pattern staging"));
 
-    // ifftshift of the MATLAB-layout propagator (mex_main.cu:80)
+    // ifftshift of the MATLAB-layout propagator (mex_main.cu:80), stored transposed ([x][y]) because
+    // the kernels apply it in the row pass that ends each forward transform
     std::vector<float2> pu(n2);
     const int n = c->n, h = n / 2;
     for (int y = 0; y < n; ++y)
         for (int xx = 0; xx < n; ++xx) {
             const size_t src = (size_t)((y + h) % n) * n + (xx + h) % n;
-            pu[(size_t)y * n + xx] = make_float2(prop2D[2 * src], prop2D[2 * src + 1]);
+            pu[(size_t)xx * n + y] = make_float2(prop2D[2 * src], prop2D[2 * src + 1]);
         }
     std::vector<float2> tw;
     twiddles(n, tw);

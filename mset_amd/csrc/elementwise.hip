@@ -15,12 +15,13 @@ static inline dim3 grid_for(size_t n, int per_thread = 1) {
 }
 
 // sqrt(ifftshift(M)) of every pattern (src/CUDA/mset.cu:206-210 does this per position per visit;
-// here it is done once at ingest): out[k][y][x] = sqrt(M[k][(y + N/2) % N][(x + N/2) % N])
+// here it is done once at ingest), stored transposed for the row pass that reads it:
+// out[k][x][y] = sqrt(M[k][(y + N/2) % N][(x + N/2) % N])
 __global__ void k_ingest(const float* __restrict__ M, float* __restrict__ out, int n, long long npos) {
     const long long n2 = (long long)n * n, tot = n2 * npos;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < tot; i += (long long)gridDim.x * blockDim.x) {
         const long long k = i / n2;
-        const int r = (int)(i - k * n2), y = r / n, x = r % n;
+        const int r = (int)(i - k * n2), x = r / n, y = r % n;
         const int ys = (y + n / 2) % n, xs = (x + n / 2) % n;
         out[i] = sqrtf(M[k * n2 + (long long)ys * n + xs]);
     }

@@ -1,18 +1,26 @@
 // fft_lds.hpp -- LDS-resident 2D FFT of one N x N complex plane for CDNA4 (gfx950).
 //
-// The plane lives in LDS as float2 s[N*N], row-major [y][x] (x fastest, the MATLAB dim-1 axis).
-// A 2D transform is a pass of 1D transforms along x (rows) followed by a pass along y (columns);
-// each pass is a sequence of Stockham autosort stages with radices Plan<N>::R.  One stage is done by
-// the whole workgroup: every thread loads its butterflies' R points into registers, the workgroup
-// synchronises, every thread applies twiddles and the R-point DFT in registers and writes the R
-// results back in place, and the workgroup synchronises again.  The data is in natural order after
-// every complete 1D pass, so no bit reversal pass exists.
+// The plane lives in LDS as float2 s[N * S] with row pitch S = N + 1: element (y, x) (x fastest, the
+// MATLAB dim-1 axis) is s[y * S + x].  A 2D transform is a pass of 1D transforms along y (columns)
+// and a pass along x (rows); each pass is a sequence of Stockham autosort stages with radices
+// Plan<N>::R.  In one stage every thread loads the R points of its butterflies into registers, the
+// workgroup synchronises, every thread applies twiddles and the R-point DFT in registers and writes
+// the results back in place, and the workgroup synchronises again.  Data is in natural order after
+// every complete 1D pass, so there is no bit-reversal pass.
 //
-// Element-wise work is fused into the transform instead of costing LDS sweeps of its own: a `pre`
-// functor is applied to every element as the first row stage loads it (it may also supply the value
-// itself, from global memory, when LOADG is set) and a `post` functor to every element as the last
-// column stage stores it.  Both see the element's natural-order linear index y*N + x, and consecutive
-// threads see consecutive x, so global accesses made inside the functors are coalesced.
+// LDS banks: in both passes consecutive lanes own consecutive lines (x in the column pass, y in the row
+// pass) at the same position along the line.  A column-pass access is therefore a contiguous run of
+// float2, and a row-pass access strides by the odd pitch S, which spreads 16 consecutive lanes over all
+// banks: neither pass has bank conflicts, whatever the stage stride.
+//
+// Element-wise work is fused into the transform instead of costing LDS sweeps of its own: `pre(y, x, v)`
+// is applied to every element as the first pass loads it (with LOADG the functor supplies the value
+// from global memory and LDS is not read) and `post(y, x, v)` as the last pass stores it.  A forward
+// transform (FWD_CR) runs columns then rows, an inverse (INV_RC) rows then columns.  In a column pass
+// consecutive lanes see consecutive x, so functors touching row-major [y][x] global arrays are
+// coalesced; in a row pass consecutive lanes see consecutive y, so functors there should touch arrays
+// stored transposed, [x][y].  Callers choose layouts accordingly (the propagator, for instance, is only
+// ever applied in the row pass that ends a forward transform and is kept transposed).
 //
 // Forward transforms use exp(-2 pi i / N) (MATLAB fftn), inverse ones exp(+2 pi i / N) without the
 // 1/N^2 factor (callers fold it into a post functor).  Twiddles come from a table tw[k] = W_N^k,
@@ -49,6 +57,9 @@ template <> struct Plan<64>  { static constexpr int S = 2; static constexpr int 
 template <> struct Plan<96>  { static constexpr int S = 3; static constexpr int R[3] = {8, 4, 3}; };
 template <> struct Plan<128> { static constexpr int S = 2; static constexpr int R[2] = {16, 8}; };
 template <> struct Plan<256> { static constexpr int S = 2; static constexpr int R[2] = {16, 16}; };
+
+// LDS row pitch of an N x N plane
+template <int N> struct Pitch { static constexpr int S = N + 1; };
 
 // ------------------------------------------------------------------ small DFTs in registers
 // W_16^k = cos(2 pi k/16) - i sin(2 pi k/16) for k = 0..7
@@ -118,15 +129,17 @@ template <bool INV> struct DFT<3, INV> {
 
 // ------------------------------------------------------------------ functors
 struct NoOp {
-    __device__ __forceinline__ float2 operator()(int, float2 v) const { return v; }
+    __device__ __forceinline__ float2 operator()(int, int, float2 v) const { return v; }
 };
 
 // ------------------------------------------------------------------ one Stockham stage
-// COLS: transform along y (stride N) instead of x.  NS: span (product of the radices already done).
-// FIRST: apply `pre` on load (LOADG: pre supplies the value, LDS is not read).  LAST: apply `post`.
-template <int N, int NT, int R, int NS, bool INV, bool COLS, bool FIRST, bool LOADG, bool LAST, class Pre, class Post>
+// ROWS: transform along x (lines are rows y); otherwise along y (lines are columns x).
+// NS: span (product of the radices already done).  FIRST: apply `pre` on load (LOADG: pre supplies the
+// value, LDS is not read).  LAST: apply `post` on store.
+template <int N, int NT, int R, int NS, bool INV, bool ROWS, bool FIRST, bool LOADG, bool LAST, class Pre, class Post>
 __device__ __forceinline__ void fft_stage(float2* __restrict__ s, const float2* __restrict__ tw, const Pre& pre,
                                           const Post& post) {
+    constexpr int S = Pitch<N>::S;
     constexpr int NL = N / R;                 // butterflies per line
     constexpr int NB = N * NL;                // butterflies in the pass
     constexpr int CNT = (NB + NT - 1) / NT;   // butterflies per thread
@@ -137,16 +150,15 @@ __device__ __forceinline__ void fft_stage(float2* __restrict__ s, const float2* 
     for (int c = 0; c < CNT; ++c) {
         const int b = t + c * NT;
         if (NB % NT == 0 || b < NB) {
-            const int line = COLS ? b % N : b / NL;
-            const int j = COLS ? b / N : b % NL;
+            const int line = b % N, j = b / N;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int e = j + r * NL;
-                const int idx = COLS ? e * N + line : line * N + e;
+                const int y = ROWS ? line : e, x = ROWS ? e : line;
                 float2 val;
-                if constexpr (FIRST && LOADG) val = pre(idx, make_float2(0.f, 0.f));
-                else if constexpr (FIRST) val = pre(idx, s[idx]);
-                else val = s[idx];
+                if constexpr (FIRST && LOADG) val = pre(y, x, make_float2(0.f, 0.f));
+                else if constexpr (FIRST) val = pre(y, x, s[y * S + x]);
+                else val = s[y * S + x];
                 v[c][r] = val;
             }
         }
@@ -156,8 +168,7 @@ __device__ __forceinline__ void fft_stage(float2* __restrict__ s, const float2* 
     for (int c = 0; c < CNT; ++c) {
         const int b = t + c * NT;
         if (NB % NT == 0 || b < NB) {
-            const int line = COLS ? b % N : b / NL;
-            const int j = COLS ? b / N : b % NL;
+            const int line = b % N, j = b / N;
             const int k = j % NS;
             if constexpr (NS > 1) {
 #pragma unroll
@@ -172,32 +183,39 @@ __device__ __forceinline__ void fft_stage(float2* __restrict__ s, const float2* 
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int e = base + r * NS;
-                const int idx = COLS ? e * N + line : line * N + e;
+                const int y = ROWS ? line : e, x = ROWS ? e : line;
                 float2 val = v[c][r];
-                if constexpr (LAST) val = post(idx, val);
-                s[idx] = val;
+                if constexpr (LAST) val = post(y, x, val);
+                s[y * S + x] = val;
             }
         }
     }
     __syncthreads();
 }
 
-template <int N, int NT, bool INV, bool COLS, int ST, int NS, bool FIRSTPASS, bool LOADG, bool LASTPASS, class Pre,
+template <int N, int NT, bool INV, bool ROWS, int ST, int NS, bool FIRSTPASS, bool LOADG, bool LASTPASS, class Pre,
           class Post>
 __device__ __forceinline__ void fft_pass(float2* s, const float2* tw, const Pre& pre, const Post& post) {
     if constexpr (ST < Plan<N>::S) {
         constexpr int R = Plan<N>::R[ST];
-        fft_stage<N, NT, R, NS, INV, COLS, FIRSTPASS && ST == 0, LOADG, LASTPASS && ST == Plan<N>::S - 1>(s, tw, pre,
+        fft_stage<N, NT, R, NS, INV, ROWS, FIRSTPASS && ST == 0, LOADG, LASTPASS && ST == Plan<N>::S - 1>(s, tw, pre,
                                                                                                         post);
-        fft_pass<N, NT, INV, COLS, ST + 1, NS * R, FIRSTPASS, LOADG, LASTPASS>(s, tw, pre, post);
+        fft_pass<N, NT, INV, ROWS, ST + 1, NS * R, FIRSTPASS, LOADG, LASTPASS>(s, tw, pre, post);
     }
 }
 
-// Full 2D transform of the LDS plane s with fused pre/post element operations.
-template <int N, int NT, bool INV, bool LOADG = false, class Pre = NoOp, class Post = NoOp>
-__device__ __forceinline__ void fft2d(float2* s, const float2* tw, const Pre& pre = Pre(), const Post& post = Post()) {
-    fft_pass<N, NT, INV, false, 0, 1, true, LOADG, false>(s, tw, pre, post);
-    fft_pass<N, NT, INV, true, 0, 1, false, false, true>(s, tw, pre, post);
+// Forward 2D transform: columns (pre fused, lanes along x) then rows (post fused, lanes along y).
+template <int N, int NT, bool LOADG = false, class Pre = NoOp, class Post = NoOp>
+__device__ __forceinline__ void fft2d_fwd(float2* s, const float2* tw, const Pre& pre = Pre(), const Post& post = Post()) {
+    fft_pass<N, NT, false, false, 0, 1, true, LOADG, false>(s, tw, pre, post);
+    fft_pass<N, NT, false, true, 0, 1, false, false, true>(s, tw, pre, post);
+}
+
+// Inverse 2D transform (no 1/N^2): rows (pre fused, lanes along y) then columns (post fused, lanes along x).
+template <int N, int NT, bool LOADG = false, class Pre = NoOp, class Post = NoOp>
+__device__ __forceinline__ void fft2d_inv(float2* s, const float2* tw, const Pre& pre = Pre(), const Post& post = Post()) {
+    fft_pass<N, NT, true, true, 0, 1, true, LOADG, false>(s, tw, pre, post);
+    fft_pass<N, NT, true, false, 0, 1, false, false, true>(s, tw, pre, post);
 }
 
 }  // namespace mset

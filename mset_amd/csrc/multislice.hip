@@ -21,6 +21,11 @@
 // MATLAB layout; the multislice operator commutes with circular shifts, so this equals MATLAB's
 // fftshift(fftn(ifftshift(.))) chain (STEP02_FORWARD_4D.m:32-36).  The sub-pixel shift follows
 // Func_generate_probe_wave_scan_sub_pixel.m:4-27 (grid -n/2..n/2-1, so the Nyquist bin uses -n/2).
+//
+// Global layouts (chosen so that every fused access is coalesced, see fft_lds.hpp): row-major [y][x]
+// for V, transmission planes, saved waves, gradient slabs and the probe; transposed [x][y] for the
+// arrays read in the row pass that ends a forward transform -- the propagator, sqrt(M), fftn(probe)
+// and the Fourier-domain incident wave.
 #include <hip/hip_runtime.h>
 #include "fft_lds.hpp"
 #include "kernels.hpp"
@@ -73,15 +78,18 @@ template <int N, int NT, int KIND>
 __device__ void process_position(const PosArgs& A, int k, int slot, float2* s, float2* tw, float2* phx,
                                  float2* phy, double* red) {
     constexpr int N2 = N * N;
+    constexpr int S = Pitch<N>::S;
     const int t = threadIdx.x;
     const int L = A.L;
     const float inv_n2 = 1.0f / (float)N2;
-    float2* save = A.save + (size_t)slot * (L + 2) * N2;   // slots 0..L waves, L+1 Fourier incident wave
+    float2* save = A.save + (size_t)slot * (L + 2) * N2;   // planes 0..L: waves [y][x]; L+1: Fourier psi0 [x][y]
+    float2* F0 = save + (size_t)(L + 1) * N2;
     const bool want_pos = (A.want & WANT_POS) != 0;
     const bool want_probe = (A.want & WANT_PROBE) != 0;
     const bool exact = A.grad_mode == GRAD_EXACT;
+    const float2* pT = A.prop;                              // propagator, [x][y]
 
-    // ---- sub-pixel shift factors, separable: Phi(jy, jx) = phy[jy] * phx[jx]  (double-precision phase)
+    // ---- sub-pixel shift factors, separable: Phi(y, x) = phy[y] * phx[x]  (double-precision phase)
     const double row = (double)A.xlist[k], col = (double)A.ylist[k];
     const double dxs = row - (double)A.cent, dys = col - (double)A.cent;
     for (int j = t; j < 2 * N; j += NT) {
@@ -93,55 +101,56 @@ __device__ void process_position(const PosArgs& A, int k, int slot, float2* s, f
         (j < N ? phx : phy)[jj] = make_float2((float)cs, (float)sn);
     }
     __syncthreads();
-    auto phase = [&](int idx) { return cmul(phy[idx / N], phx[idx % N]); };
+    auto phase = [&](int y, int x) { return cmul(phy[y], phx[x]); };
 
     // ---- incident wave psi0 = ifftn(fftn(probe) .* Phi)
     if constexpr (KIND == K_PLAIN) {
         const float2* probe = A.probe;
-        fft2d<N, NT, false, true>(s, tw, [&](int idx, float2) { return probe[idx]; },
-                                  [&](int idx, float2 v) {
-                                      const float2 f = cmul(v, phase(idx));
-                                      save[(size_t)(L + 1) * N2 + idx] = f;
-                                      return f;
-                                  });
-        fft2d<N, NT, true>(s, tw, NoOp(), [&](int idx, float2 v) {
+        fft2d_fwd<N, NT, true>(s, tw, [&](int y, int x, float2) { return probe[y * N + x]; },
+                               [&](int y, int x, float2 v) {
+                                   const float2 f = cmul(v, phase(y, x));
+                                   F0[x * N + y] = f;
+                                   return f;
+                               });
+        fft2d_inv<N, NT>(s, tw, NoOp(), [&](int y, int x, float2 v) {
             const float2 w = cscale(v, inv_n2);
-            save[idx] = w;
+            save[y * N + x] = w;
             return w;
         });
     } else {
-        const float2* pf = A.probeF;
-        fft2d<N, NT, true, true>(s, tw,
-                                 [&](int idx, float2) {
-                                     const float2 f = cmul(pf[idx], phase(idx));
-                                     if (KIND == K_BATCH) save[(size_t)(L + 1) * N2 + idx] = f;
-                                     return f;
-                                 },
-                                 [&](int idx, float2 v) {
-                                     const float2 w = cscale(v, inv_n2);
-                                     if (KIND == K_BATCH) save[idx] = w;
-                                     return w;
-                                 });
+        const float2* pfT = A.probeF;                       // fftn(probe), [x][y]
+        fft2d_inv<N, NT, true>(s, tw,
+                               [&](int y, int x, float2) {
+                                   const float2 f = cmul(pfT[x * N + y], phase(y, x));
+                                   if (KIND == K_BATCH) F0[x * N + y] = f;
+                                   return f;
+                               },
+                               [&](int y, int x, float2 v) {
+                                   const float2 w = cscale(v, inv_n2);
+                                   if (KIND == K_BATCH) save[y * N + x] = w;
+                                   return w;
+                               });
     }
 
     // ---- forward sweep: psi_a = ifftn(fftn(psi_{a-1} .* t_a) .* p)
-    const float2* prop = A.prop;
     for (int a = 0; a < L; ++a) {
         const int z0 = a * A.bin, z1 = min((a + 1) * A.bin, A.nz);
         if constexpr (KIND == K_PLAIN) {
             const float* V = A.V;
-            fft2d<N, NT, false>(s, tw,
-                                [&](int idx, float2 v) { return cmul(v, trans_from_volume(V, N2, z0, z1, idx, A.dx, A.sigma)); },
-                                [&](int idx, float2 v) { return cmul(v, prop[idx]); });
+            fft2d_fwd<N, NT>(s, tw,
+                             [&](int y, int x, float2 v) {
+                                 return cmul(v, trans_from_volume(V, N2, z0, z1, y * N + x, A.dx, A.sigma));
+                             },
+                             [&](int y, int x, float2 v) { return cmul(v, pT[x * N + y]); });
         } else {
             const float2* ta = A.trans + (size_t)a * N2;
-            fft2d<N, NT, false>(s, tw, [&](int idx, float2 v) { return cmul(v, ta[idx]); },
-                                [&](int idx, float2 v) { return cmul(v, prop[idx]); });
+            fft2d_fwd<N, NT>(s, tw, [&](int y, int x, float2 v) { return cmul(v, ta[y * N + x]); },
+                             [&](int y, int x, float2 v) { return cmul(v, pT[x * N + y]); });
         }
         float2* sv = save + (size_t)(a + 1) * N2;
-        fft2d<N, NT, true>(s, tw, NoOp(), [&](int idx, float2 v) {
+        fft2d_inv<N, NT>(s, tw, NoOp(), [&](int y, int x, float2 v) {
             const float2 w = cscale(v, inv_n2);
-            if (KIND != K_FWD) sv[idx] = w;
+            if (KIND != K_FWD) sv[y * N + x] = w;
             return w;
         });
     }
@@ -149,17 +158,17 @@ __device__ void process_position(const PosArgs& A, int k, int slot, float2* s, f
     // ---- exit wave, intensity, amplitude residual (STEP02_FORWARD_4D.m:36-49), r .* conj(p) fused
     double err = 0.0;
     {
-        const float* M = A.sqrtM + (size_t)k * N2;
+        const float* MT = A.sqrtM + (size_t)k * N2;         // sqrt(M), [x][y]
         float* Iout = A.inten ? A.inten + (size_t)k * N2 : nullptr;
-        fft2d<N, NT, false>(s, tw, NoOp(), [&](int idx, float2 v) {
+        fft2d_fwd<N, NT>(s, tw, NoOp(), [&](int y, int x, float2 v) {
             const float I = v.x * v.x + v.y * v.y;
-            if (KIND == K_FWD && Iout) Iout[((idx / N + N / 2) % N) * N + (idx % N + N / 2) % N] = I;
-            const float m = M[idx];
+            if (KIND == K_FWD && Iout) Iout[((y + N / 2) % N) * N + (x + N / 2) % N] = I;
+            const float m = MT[x * N + y];
             const float d = m - sqrtf(I);
             err += (double)(d * d);
             if (KIND == K_FWD) return v;
             const float f = 1.0f - m / sqrtf(I + 1e-30f);
-            return cmulc(make_float2(v.x * f, v.y * f), prop[idx]);
+            return cmulc(make_float2(v.x * f, v.y * f), pT[x * N + y]);
         });
     }
     err = block_sum<NT>(err, red);
@@ -173,27 +182,29 @@ __device__ void process_position(const PosArgs& A, int k, int slot, float2* s, f
         if constexpr (KIND == K_PLAIN) {
             float* V = A.V;
             const float s1 = A.s1;
-            fft2d<N, NT, true>(s, tw, NoOp(), [&](int idx, float2 v) {
+            fft2d_inv<N, NT>(s, tw, NoOp(), [&](int y, int x, float2 v) {
+                const int i = y * N + x;
                 const float2 b = cscale(v, inv_n2);
-                const float2 tc = cconj(trans_from_volume(V, N2, z0, z1, idx, A.dx, A.sigma));
-                const float2 q = cmul(tc, cmulc(b, ws[idx]));
+                const float2 tc = cconj(trans_from_volume(V, N2, z0, z1, i, A.dx, A.sigma));
+                const float2 q = cmul(tc, cmulc(b, ws[i]));
                 const float gr = -q.y;                       // Re(1i * conj(t) * conj(psi) * b)
-                for (int z = z0; z < z1; ++z) V[(size_t)z * N2 + idx] += s1 * gr;
+                for (int z = z0; z < z1; ++z) V[(size_t)z * N2 + i] += s1 * gr;
                 return cmul(b, tc);
             });
         } else {
             const float2* ta = A.trans + (size_t)a * N2;
             float* g = A.gslab + ((size_t)slot * L + a) * N2;
-            fft2d<N, NT, true>(s, tw, NoOp(), [&](int idx, float2 v) {
+            fft2d_inv<N, NT>(s, tw, NoOp(), [&](int y, int x, float2 v) {
+                const int i = y * N + x;
                 const float2 b = cscale(v, inv_n2);
-                const float2 tc = cconj(ta[idx]);
-                const float2 q = cmul(tc, cmulc(b, ws[idx]));
-                g[idx] += -q.y;
+                const float2 tc = cconj(ta[i]);
+                const float2 q = cmul(tc, cmulc(b, ws[i]));
+                g[i] += -q.y;
                 return cmul(b, tc);
             });
         }
-        if (a > 0) fft2d<N, NT, false>(s, tw, NoOp(), [&](int idx, float2 v) { return cmulc(v, prop[idx]); });
-        else fft2d<N, NT, false>(s, tw);
+        if (a > 0) fft2d_fwd<N, NT>(s, tw, NoOp(), [&](int y, int x, float2 v) { return cmulc(v, pT[x * N + y]); });
+        else fft2d_fwd<N, NT>(s, tw);
     }
     // s now holds fftn(back_wave_f) (natural-order Fourier domain)
 
@@ -201,7 +212,10 @@ __device__ void process_position(const PosArgs& A, int k, int slot, float2* s, f
     if constexpr (KIND == K_BATCH) {
         if (want_probe) {
             float2* ps = A.pslab + (size_t)slot * N2;
-            for (int idx = t; idx < N2; idx += NT) ps[idx] = cadd(ps[idx], cmulc(s[idx], phase(idx)));
+            for (int i = t; i < N2; i += NT) {
+                const int y = i / N, x = i % N;
+                ps[i] = cadd(ps[i], cmulc(s[y * S + x], phase(y, x)));
+            }
         }
     }
 
@@ -211,32 +225,31 @@ __device__ void process_position(const PosArgs& A, int k, int slot, float2* s, f
     if (want_pos) {
         if (exact) {
             // Parseval: sum_x bw conj(-dpsi0/drow) = (1/N^2) sum_q B(q) conj(Psi0(q)) (-2 pi i g/N)
-            const float2* F0 = save + (size_t)(L + 1) * N2;
-            for (int idx = t; idx < N2; idx += NT) {
-                const float2 c = cmulc(s[idx], F0[idx]);       // B conj(Psi0)
-                const double gx = shift_grid<N>(idx % N), gy = shift_grid<N>(idx / N);
+            for (int i = t; i < N2; i += NT) {
+                const int y = i % N, x = i / N;                // lanes along y: F0 is [x][y]
+                const float2 c = cmulc(s[y * S + x], F0[i]);  // B conj(Psi0)
                 // Re(c * (-i) * 2 pi g / N) = Im(c) * 2 pi g / N
-                dr += (double)c.y * gx;
-                dc += (double)c.y * gy;
+                dr += (double)c.y * shift_grid<N>(x);
+                dc += (double)c.y * shift_grid<N>(y);
             }
             const double f = 2.0 * M_PI / (double)N / (double)N2;
             dr = block_sum<NT>(dr, red) * f;
             dc = block_sum<NT>(dc, red) * f;
         } else {
             // back_wave_f = ifftn(B); MATLAB gradient() of conj(psi0), one-sided at the edges
-            fft2d<N, NT, true>(s, tw, NoOp(), [&](int, float2 v) { return cscale(v, inv_n2); });
+            fft2d_inv<N, NT>(s, tw, NoOp(), [&](int, int, float2 v) { return cscale(v, inv_n2); });
             have_bw = true;
             const float2* p0 = save;
-            for (int idx = t; idx < N2; idx += NT) {
-                const int x = idx % N, y = idx / N;
-                const float2 bw = s[idx];
+            for (int i = t; i < N2; i += NT) {
+                const int x = i % N, y = i / N;
+                const float2 bw = s[y * S + x];
                 float2 gx, gy;
-                if (x == 0) gx = csub(p0[idx + 1], p0[idx]);
-                else if (x == N - 1) gx = csub(p0[idx], p0[idx - 1]);
-                else gx = cscale(csub(p0[idx + 1], p0[idx - 1]), 0.5f);
-                if (y == 0) gy = csub(p0[idx + N], p0[idx]);
-                else if (y == N - 1) gy = csub(p0[idx], p0[idx - N]);
-                else gy = cscale(csub(p0[idx + N], p0[idx - N]), 0.5f);
+                if (x == 0) gx = csub(p0[i + 1], p0[i]);
+                else if (x == N - 1) gx = csub(p0[i], p0[i - 1]);
+                else gx = cscale(csub(p0[i + 1], p0[i - 1]), 0.5f);
+                if (y == 0) gy = csub(p0[i + N], p0[i]);
+                else if (y == N - 1) gy = csub(p0[i], p0[i - N]);
+                else gy = cscale(csub(p0[i + N], p0[i - N]), 0.5f);
                 // Re(conj(g) * bw)
                 dr += (double)(gx.x * bw.x + gx.y * bw.y);
                 dc += (double)(gy.x * bw.x + gy.y * bw.y);
@@ -256,21 +269,17 @@ __device__ void process_position(const PosArgs& A, int k, int slot, float2* s, f
         if (want_probe) {
             float2* probe = A.probe;
             const float s2 = A.s2;
+            auto upd = [&](int y, int x, float2 v) {
+                const float2 w = cscale(v, inv_n2);
+                probe[y * N + x] = csub(probe[y * N + x], cscale(w, s2));
+                return w;
+            };
             if (have_bw) {
                 // MATLAB shifts back_wave_f: fftn(bw) .* conj(Phi) -> ifftn
-                fft2d<N, NT, false>(s, tw, NoOp(), [&](int idx, float2 v) { return cmulc(v, phase(idx)); });
-                fft2d<N, NT, true>(s, tw, NoOp(), [&](int idx, float2 v) {
-                    const float2 w = cscale(v, inv_n2);
-                    probe[idx] = csub(probe[idx], cscale(w, s2));
-                    return w;
-                });
+                fft2d_fwd<N, NT>(s, tw, NoOp(), [&](int y, int x, float2 v) { return cmulc(v, phase(y, x)); });
+                fft2d_inv<N, NT>(s, tw, NoOp(), upd);
             } else {
-                fft2d<N, NT, true>(s, tw, [&](int idx, float2 v) { return cmulc(v, phase(idx)); },
-                                   [&](int idx, float2 v) {
-                                       const float2 w = cscale(v, inv_n2);
-                                       probe[idx] = csub(probe[idx], cscale(w, s2));
-                                       return w;
-                                   });
+                fft2d_inv<N, NT>(s, tw, [&](int y, int x, float2 v) { return cmulc(v, phase(y, x)); }, upd);
             }
         }
         if (t == 0 && want_pos) {
@@ -285,12 +294,15 @@ template <int N, int NT> __device__ __forceinline__ void load_twiddles(float2* t
     __syncthreads();
 }
 
+#define MSET_LDS(N, NT)                                 \
+    __shared__ float2 s[N * Pitch<N>::S];               \
+    __shared__ float2 tw[N], phx[N], phy[N];            \
+    __shared__ double red[(NT + 63) / 64];
+
 template <int N>
 __global__ __launch_bounds__(Cfg<N>::NT) void k_plain(PosArgs A, int k) {
     constexpr int NT = Cfg<N>::NT;
-    __shared__ float2 s[N * N];
-    __shared__ float2 tw[N], phx[N], phy[N];
-    __shared__ double red[(NT + 63) / 64];
+    MSET_LDS(N, NT)
     load_twiddles<N, NT>(tw, A.tw);
     process_position<N, NT, K_PLAIN>(A, k, 0, s, tw, phx, phy, red);
 }
@@ -298,9 +310,7 @@ __global__ __launch_bounds__(Cfg<N>::NT) void k_plain(PosArgs A, int k) {
 template <int N>
 __global__ __launch_bounds__(Cfg<N>::NT) void k_batch(PosArgs A) {
     constexpr int NT = Cfg<N>::NT;
-    __shared__ float2 s[N * N];
-    __shared__ float2 tw[N], phx[N], phy[N];
-    __shared__ double red[(NT + 63) / 64];
+    MSET_LDS(N, NT)
     load_twiddles<N, NT>(tw, A.tw);
     for (int p = blockIdx.x; p < A.count; p += gridDim.x) {
         const int k = A.order ? A.order[A.first + p] : A.first + p;
@@ -312,9 +322,7 @@ __global__ __launch_bounds__(Cfg<N>::NT) void k_batch(PosArgs A) {
 template <int N>
 __global__ __launch_bounds__(Cfg<N>::NT) void k_fwd(PosArgs A) {
     constexpr int NT = Cfg<N>::NT;
-    __shared__ float2 s[N * N];
-    __shared__ float2 tw[N], phx[N], phy[N];
-    __shared__ double red[(NT + 63) / 64];
+    MSET_LDS(N, NT)
     load_twiddles<N, NT>(tw, A.tw);
     for (int p = blockIdx.x; p < A.count; p += gridDim.x) {
         const int k = A.order ? A.order[A.first + p] : A.first + p;
@@ -323,29 +331,36 @@ __global__ __launch_bounds__(Cfg<N>::NT) void k_fwd(PosArgs A) {
     }
 }
 
-// fftn (dir = 0) or ifftn with 1/N^2 (dir = 1) of one plane, optionally scaled and added:
-// out = beta * out + alpha * T(in)   (beta = 0: plain store)
+// One plane, row-major in and out unless noted:
+//   dir 0: out^T = fftn(in) written transposed ([x][y]), alpha/beta ignored;
+//   dir 1: out = beta * out + alpha * ifftn(in) (with 1/N^2; beta = 0: plain store).
 template <int N>
 __global__ __launch_bounds__(Cfg<N>::NT) void k_plane_fft(const float2* in, float2* out, const float2* twg, int dir,
                                                          float alpha, float beta) {
     constexpr int NT = Cfg<N>::NT;
-    __shared__ float2 s[N * N];
+    constexpr int S = Pitch<N>::S;
+    __shared__ float2 s[N * S];
     __shared__ float2 tw[N];
     load_twiddles<N, NT>(tw, twg);
-    const float sc = dir ? alpha / (float)(N * N) : alpha;
-    auto ld = [&](int idx, float2) { return in[idx]; };
-    auto st = [&](int idx, float2 v) {
-        const float2 w = cscale(v, sc);
-        out[idx] = beta == 0.f ? w : cadd(cscale(out[idx], beta), w);
-        return v;
-    };
-    if (dir) fft2d<N, NT, true, true>(s, tw, ld, st);
-    else fft2d<N, NT, false, true>(s, tw, ld, st);
+    for (int i = threadIdx.x; i < N * N; i += NT) s[(i / N) * S + i % N] = in[i];
+    __syncthreads();
+    if (dir == 0) {
+        fft2d_fwd<N, NT>(s, tw);
+        for (int i = threadIdx.x; i < N * N; i += NT) {
+            const int y = i % N, x = i / N;                    // lanes along y: coalesced transposed store
+            out[i] = s[y * S + x];
+        }
+    } else {
+        fft2d_inv<N, NT>(s, tw);
+        const float sc = alpha / (float)(N * N);
+        for (int i = threadIdx.x; i < N * N; i += NT) {
+            const float2 w = cscale(s[(i / N) * S + i % N], sc);
+            out[i] = beta == 0.f ? w : cadd(cscale(out[i], beta), w);
+        }
+    }
 }
 
 // ------------------------------------------------------------------ host launchers
-template <int N> static int nt_of() { return Cfg<N>::NT; }
-
 #define MSET_DISPATCH(N, BODY)                                              \
     switch (N) {                                                            \
         case 16: { constexpr int NN = 16; BODY; } break;                    \
@@ -363,18 +378,16 @@ bool plane_size_supported(int n) {
 }
 
 int position_threads(int n) {
-    int r = 0;
     switch (n) {
-        case 16: r = Cfg<16>::NT; break;
-        case 24: r = Cfg<24>::NT; break;
-        case 32: r = Cfg<32>::NT; break;
-        case 48: r = Cfg<48>::NT; break;
-        case 64: r = Cfg<64>::NT; break;
-        case 96: r = Cfg<96>::NT; break;
-        case 128: r = Cfg<128>::NT; break;
-        default: r = 0;
+        case 16: return Cfg<16>::NT;
+        case 24: return Cfg<24>::NT;
+        case 32: return Cfg<32>::NT;
+        case 48: return Cfg<48>::NT;
+        case 64: return Cfg<64>::NT;
+        case 96: return Cfg<96>::NT;
+        case 128: return Cfg<128>::NT;
+        default: return 0;
     }
-    return r;
 }
 
 hipError_t launch_plain(int n, const PosArgs& A, int k, hipStream_t st) {

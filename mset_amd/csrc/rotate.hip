@@ -14,7 +14,8 @@
 //
 // A pass is a batched 1D FFT along one axis of the complex work volume: each workgroup loads a tile
 // of TL lines into LDS (for the y and z axes a tile is TL consecutive x positions, so loads stay
-// coalesced), runs the Stockham stages of fft_lds.hpp's radix plans, and writes the tile back with
+// coalesced), runs the Stockham stages of fft_lds.hpp's radix plans with consecutive lanes on
+// consecutive lines (so LDS accesses are free of bank conflicts), and writes the tile back with
 // the 1/N of an inverse transform, the phase ramp and the real part fused into the store.
 #include <hip/hip_runtime.h>
 
@@ -50,8 +51,7 @@ __device__ __forceinline__ void line_stage(float2* __restrict__ s, const float2*
     for (int c = 0; c < CNT; ++c) {
         const int b = t + c * NT;
         if (NB % NT == 0 || b < NB) {
-            const int line = ES == 1 ? b / NL : b % NLINES;
-            const int j = ES == 1 ? b % NL : b / NLINES;
+            const int line = b % NLINES, j = b / NLINES;   // lanes along lines: conflict-free with odd LS
 #pragma unroll
             for (int r = 0; r < R; ++r) v[c][r] = s[line * LS + (j + r * NL) * ES];
         }
@@ -61,8 +61,7 @@ __device__ __forceinline__ void line_stage(float2* __restrict__ s, const float2*
     for (int c = 0; c < CNT; ++c) {
         const int b = t + c * NT;
         if (NB % NT == 0 || b < NB) {
-            const int line = ES == 1 ? b / NL : b % NLINES;
-            const int j = ES == 1 ? b % NL : b / NLINES;
+            const int line = b % NLINES, j = b / NLINES;
             const int k = j % NS;
             if constexpr (NS > 1) {
 #pragma unroll
@@ -123,7 +122,10 @@ __device__ __forceinline__ float2 shear_phase(const PassArgs& P, int jx, int jy,
 
 template <int N, int AXIS, int TL, int NT>
 __global__ __launch_bounds__(NT) void k_axis_pass(PassArgs P) {
-    __shared__ float2 s[N * TL];
+    // x lines are stored one per LDS row with an odd pitch (N + 1); y / z tiles as [element][line]
+    constexpr int ES = AXIS == 0 ? 1 : TL;
+    constexpr int LS = AXIS == 0 ? N + 1 : 1;
+    __shared__ float2 s[AXIS == 0 ? (N + 1) * TL : N * TL];
     __shared__ float2 tw[N];
     for (int i = threadIdx.x; i < N; i += NT) tw[i] = P.tw[i];
     const size_t sxy = (size_t)P.nx * P.ny;
@@ -152,8 +154,6 @@ __global__ __launch_bounds__(NT) void k_axis_pass(PassArgs P) {
             return (size_t)jz * sxy + (size_t)jy * P.nx + jx;
         }
     };
-    constexpr int ES = AXIS == 0 ? 1 : TL;
-    constexpr int LS = AXIS == 0 ? N : 1;
     for (int i = threadIdx.x; i < N * TL; i += NT) {
         const int l = AXIS == 0 ? i / N : i % TL, e = AXIS == 0 ? i % N : i / TL;
         bool ok;
