@@ -62,13 +62,27 @@ template <> struct Plan<256> { static constexpr int S = 2; static constexpr int 
 template <int N> struct Pitch { static constexpr int S = N + 1; };
 
 // ------------------------------------------------------------------ small DFTs in registers
-// W_16^k = cos(2 pi k/16) - i sin(2 pi k/16) for k = 0..7
-__device__ __forceinline__ float2 w16(int k, bool inv) {
-    constexpr float c[8] = {1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f,
-                            0.0f, -0.38268343236508977f, -0.70710678118654752f, -0.92387953251128674f};
-    constexpr float s[8] = {0.0f, 0.38268343236508977f, 0.70710678118654752f, 0.92387953251128674f,
-                            1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f};
-    return make_float2(c[k], inv ? s[k] : -s[k]);
+// W_64^j = cos(2 pi j/64) - i sin(2 pi j/64) for j = 0..31; W_R^k = W_64^(k 64/R)
+__device__ __forceinline__ float2 w64(int j, bool inv) {
+    constexpr float c[32] = {
+        1.0f, 0.99518472667219693f, 0.98078528040323043f, 0.95694033573220882f, 0.92387953251128674f,
+        0.88192126434835505f, 0.83146961230254524f, 0.77301045336273699f, 0.70710678118654757f,
+        0.63439328416364549f, 0.55557023301960229f, 0.47139673682599781f, 0.38268343236508984f,
+        0.29028467725446233f, 0.19509032201612833f, 0.09801714032956077f, 0.0f, -0.098017140329560645f,
+        -0.19509032201612819f, -0.29028467725446216f, -0.38268343236508973f, -0.4713967368259977f,
+        -0.55557023301960196f, -0.63439328416364538f, -0.70710678118654746f, -0.77301045336273699f,
+        -0.83146961230254535f, -0.88192126434835494f, -0.92387953251128674f, -0.95694033573220882f,
+        -0.98078528040323043f, -0.99518472667219682f};
+    constexpr float s[32] = {
+        0.0f, 0.098017140329560604f, 0.19509032201612825f, 0.29028467725446233f, 0.38268343236508978f,
+        0.47139673682599764f, 0.55557023301960218f, 0.63439328416364549f, 0.70710678118654746f,
+        0.77301045336273699f, 0.83146961230254524f, 0.88192126434835494f, 0.92387953251128674f,
+        0.95694033573220894f, 0.98078528040323043f, 0.99518472667219682f, 1.0f, 0.99518472667219693f,
+        0.98078528040323043f, 0.95694033573220894f, 0.92387953251128674f, 0.88192126434835505f,
+        0.83146961230254546f, 0.7730104533627371f, 0.70710678118654757f, 0.63439328416364549f,
+        0.55557023301960218f, 0.47139673682599786f, 0.38268343236508989f, 0.29028467725446239f,
+        0.19509032201612861f, 0.098017140329560826f};
+    return make_float2(c[j], inv ? s[j] : -s[j]);
 }
 
 template <int R, bool INV> struct DFT;
@@ -92,7 +106,7 @@ template <bool INV> struct DFT<4, INV> {
     }
 };
 
-// radix-2 split of an R-point DFT into two R/2-point DFTs (R = 8, 16)
+// radix-2 split of an R-point DFT into two R/2-point DFTs (R = 8, 16, 32)
 template <int R, bool INV> struct DFTsplit {
     __device__ __forceinline__ static void run(float2* v) {
         float2 e[R / 2], o[R / 2];
@@ -105,7 +119,7 @@ template <int R, bool INV> struct DFTsplit {
             float2 t;
             if (k == 0) t = o[0];
             else if (2 * k == R / 2) t = mul_mi<INV>(o[k]);
-            else t = cmul(o[k], w16(k * (16 / R), INV));
+            else t = cmul(o[k], w64(k * (64 / R), INV));
             v[k] = cadd(e[k], t);
             v[k + R / 2] = csub(e[k], t);
         }
@@ -113,6 +127,7 @@ template <int R, bool INV> struct DFTsplit {
 };
 template <bool INV> struct DFT<8, INV> : DFTsplit<8, INV> {};
 template <bool INV> struct DFT<16, INV> : DFTsplit<16, INV> {};
+template <bool INV> struct DFT<32, INV> : DFTsplit<32, INV> {};
 
 template <bool INV> struct DFT<3, INV> {
     __device__ __forceinline__ static void run(float2* v) {
@@ -131,6 +146,28 @@ template <bool INV> struct DFT<3, INV> {
 struct NoOp {
     __device__ __forceinline__ float2 operator()(int, int, float2 v) const { return v; }
 };
+
+// An element functor split in two phases: load(y, x) fetches the element's global operands, apply(y, x,
+// v, op) combines them with the element.  An engine that holds many elements per thread issues the loads
+// of G elements back to back before the first apply, so their memory latencies overlap instead of
+// adding up.  Called as an ordinary functor, it runs both phases for one element.
+template <int G_, class Ld, class Ap> struct Staged {
+    static constexpr int G = G_;
+    Ld ld;
+    Ap ap;
+    __device__ __forceinline__ auto load(int y, int x) const { return ld(y, x); }
+    template <class V, class Op> __device__ __forceinline__ auto apply(int y, int x, V&& v, const Op& o) const {
+        return ap(y, x, v, o);
+    }
+    template <class V> __device__ __forceinline__ auto operator()(int y, int x, V&& v) const {
+        return ap(y, x, v, ld(y, x));
+    }
+};
+template <int G, class Ld, class Ap> __device__ __forceinline__ Staged<G, Ld, Ap> staged(Ld ld, Ap ap) {
+    return Staged<G, Ld, Ap>{ld, ap};
+}
+template <class F> struct IsStaged { static constexpr bool value = false; };
+template <int G, class Ld, class Ap> struct IsStaged<Staged<G, Ld, Ap>> { static constexpr bool value = true; };
 
 // ------------------------------------------------------------------ one Stockham stage
 // ROWS: transform along x (lines are rows y); otherwise along y (lines are columns x).
@@ -217,5 +254,43 @@ __device__ __forceinline__ void fft2d_inv(float2* s, const float2* tw, const Pre
     fft_pass<N, NT, true, true, 0, 1, true, LOADG, false>(s, tw, pre, post);
     fft_pass<N, NT, true, false, 0, 1, false, false, true>(s, tw, pre, post);
 }
+
+// The LDS-stage transforms behind the engine interface shared with the register-resident engine:
+// fwd / inv with (y, x, value) functors, each_R / each_C to visit the plane after a forward / inverse
+// transform (lanes along y, resp. along x, so [x][y] resp. [y][x] global arrays stay coalesced).
+template <int N, int NT_> struct LdsEngine {
+    static constexpr int NT = NT_;
+    static constexpr int S = Pitch<N>::S;
+    static constexpr int LDS_ELEMS = N * S;
+    float2* s;
+    const float2* tw;
+
+    __device__ __forceinline__ void init(float2* lds, const float2* lds_tw) {
+        s = lds;
+        tw = lds_tw;
+    }
+    template <bool LOADG = false, class Pre = NoOp, class Post = NoOp>
+    __device__ __forceinline__ void fwd(const Pre& pre = Pre(), const Post& post = Post()) {
+        fft2d_fwd<N, NT, LOADG>(s, tw, pre, post);
+    }
+    template <bool LOADG = false, class Pre = NoOp, class Post = NoOp>
+    __device__ __forceinline__ void inv(const Pre& pre = Pre(), const Post& post = Post()) {
+        fft2d_inv<N, NT, LOADG>(s, tw, pre, post);
+    }
+    template <class F> __device__ __forceinline__ void each_R(const F& f) {
+        for (int i = threadIdx.x; i < N * N; i += NT) {
+            const int y = i % N, x = i / N;
+            f(y, x, s[y * S + x]);
+        }
+        __syncthreads();
+    }
+    template <class F> __device__ __forceinline__ void each_C(const F& f) {
+        for (int i = threadIdx.x; i < N * N; i += NT) {
+            const int y = i / N, x = i % N;
+            f(y, x, s[y * S + x]);
+        }
+        __syncthreads();
+    }
+};
 
 }  // namespace mset

@@ -28,18 +28,20 @@
 // and the Fourier-domain incident wave.
 #include <hip/hip_runtime.h>
 #include "fft_lds.hpp"
+#include "fft_reg.hpp"
 #include "kernels.hpp"
 
 namespace mset {
 
+// plane engine and workgroup size per plane size: register-resident at 128, LDS stages elsewhere
 template <int N> struct Cfg;
-template <> struct Cfg<16>  { static constexpr int NT = 64; };
-template <> struct Cfg<24>  { static constexpr int NT = 64; };
-template <> struct Cfg<32>  { static constexpr int NT = 64; };
-template <> struct Cfg<48>  { static constexpr int NT = 192; };
-template <> struct Cfg<64>  { static constexpr int NT = 256; };
-template <> struct Cfg<96>  { static constexpr int NT = 576; };
-template <> struct Cfg<128> { static constexpr int NT = 1024; };
+template <> struct Cfg<16>  { static constexpr int NT = 64;  using Eng = LdsEngine<16, 64>; };
+template <> struct Cfg<24>  { static constexpr int NT = 64;  using Eng = LdsEngine<24, 64>; };
+template <> struct Cfg<32>  { static constexpr int NT = 64;  using Eng = LdsEngine<32, 64>; };
+template <> struct Cfg<48>  { static constexpr int NT = 192; using Eng = LdsEngine<48, 192>; };
+template <> struct Cfg<64>  { static constexpr int NT = 256; using Eng = LdsEngine<64, 256>; };
+template <> struct Cfg<96>  { static constexpr int NT = 576; using Eng = LdsEngine<96, 576>; };
+template <> struct Cfg<128> { static constexpr int NT = 512; using Eng = RegEngine<128>; };
 
 enum { K_PLAIN = 0, K_BATCH = 1, K_FWD = 2 };
 
@@ -74,11 +76,15 @@ __device__ __forceinline__ float2 trans_from_volume(const float* __restrict__ V,
     return make_float2(cs, sn);
 }
 
-template <int N, int NT, int KIND>
-__device__ void process_position(const PosArgs& A, int k, int slot, float2* s, float2* tw, float2* phx,
-                                 float2* phy, double* red) {
+// global operands of one element of the adjoint step: transmission, saved wave, gradient slab
+struct AdjOp {
+    float2 t, w;
+    float g;
+};
+
+template <int N, int NT, int KIND, class Eng>
+__device__ void process_position(const PosArgs& A, int k, int slot, Eng& E, float2* phx, float2* phy, double* red) {
     constexpr int N2 = N * N;
-    constexpr int S = Pitch<N>::S;
     const int t = threadIdx.x;
     const int L = A.L;
     const float inv_n2 = 1.0f / (float)N2;
@@ -106,25 +112,27 @@ __device__ void process_position(const PosArgs& A, int k, int slot, float2* s, f
     // ---- incident wave psi0 = ifftn(fftn(probe) .* Phi)
     if constexpr (KIND == K_PLAIN) {
         const float2* probe = A.probe;
-        fft2d_fwd<N, NT, true>(s, tw, [&](int y, int x, float2) { return probe[y * N + x]; },
-                               [&](int y, int x, float2 v) {
-                                   const float2 f = cmul(v, phase(y, x));
-                                   F0[x * N + y] = f;
-                                   return f;
-                               });
-        fft2d_inv<N, NT>(s, tw, NoOp(), [&](int y, int x, float2 v) {
+        E.template fwd<true>(staged<16>([&](int y, int x) { return probe[y * N + x]; },
+                                        [&](int, int, float2, float2 o) { return o; }),
+                             [&](int y, int x, float2 v) {
+                                 const float2 f = cmul(v, phase(y, x));
+                                 F0[x * N + y] = f;
+                                 return f;
+                             });
+        E.inv(NoOp(), [&](int y, int x, float2 v) {
             const float2 w = cscale(v, inv_n2);
             save[y * N + x] = w;
             return w;
         });
     } else {
         const float2* pfT = A.probeF;                       // fftn(probe), [x][y]
-        fft2d_inv<N, NT, true>(s, tw,
-                               [&](int y, int x, float2) {
-                                   const float2 f = cmul(pfT[x * N + y], phase(y, x));
-                                   if (KIND == K_BATCH) F0[x * N + y] = f;
-                                   return f;
-                               },
+        E.template inv<true>(
+                               staged<16>([&](int y, int x) { return pfT[x * N + y]; },
+                                          [&](int y, int x, float2, float2 o) {
+                                              const float2 f = cmul(o, phase(y, x));
+                                              if (KIND == K_BATCH) F0[x * N + y] = f;
+                                              return f;
+                                          }),
                                [&](int y, int x, float2 v) {
                                    const float2 w = cscale(v, inv_n2);
                                    if (KIND == K_BATCH) save[y * N + x] = w;
@@ -137,18 +145,20 @@ __device__ void process_position(const PosArgs& A, int k, int slot, float2* s, f
         const int z0 = a * A.bin, z1 = min((a + 1) * A.bin, A.nz);
         if constexpr (KIND == K_PLAIN) {
             const float* V = A.V;
-            fft2d_fwd<N, NT>(s, tw,
-                             [&](int y, int x, float2 v) {
+            E.fwd([&](int y, int x, float2 v) {
                                  return cmul(v, trans_from_volume(V, N2, z0, z1, y * N + x, A.dx, A.sigma));
                              },
-                             [&](int y, int x, float2 v) { return cmul(v, pT[x * N + y]); });
+                             staged<16>([&](int y, int x) { return pT[x * N + y]; },
+                                        [&](int, int, float2 v, float2 o) { return cmul(v, o); }));
         } else {
             const float2* ta = A.trans + (size_t)a * N2;
-            fft2d_fwd<N, NT>(s, tw, [&](int y, int x, float2 v) { return cmul(v, ta[y * N + x]); },
-                             [&](int y, int x, float2 v) { return cmul(v, pT[x * N + y]); });
+            E.fwd(staged<16>([&](int y, int x) { return ta[y * N + x]; },
+                             [&](int, int, float2 v, float2 o) { return cmul(v, o); }),
+                  staged<16>([&](int y, int x) { return pT[x * N + y]; },
+                             [&](int, int, float2 v, float2 o) { return cmul(v, o); }));
         }
         float2* sv = save + (size_t)(a + 1) * N2;
-        fft2d_inv<N, NT>(s, tw, NoOp(), [&](int y, int x, float2 v) {
+        E.inv(NoOp(), [&](int y, int x, float2 v) {
             const float2 w = cscale(v, inv_n2);
             if (KIND != K_FWD) sv[y * N + x] = w;
             return w;
@@ -160,16 +170,22 @@ __device__ void process_position(const PosArgs& A, int k, int slot, float2* s, f
     {
         const float* MT = A.sqrtM + (size_t)k * N2;         // sqrt(M), [x][y]
         float* Iout = A.inten ? A.inten + (size_t)k * N2 : nullptr;
-        fft2d_fwd<N, NT>(s, tw, NoOp(), [&](int y, int x, float2 v) {
-            const float I = v.x * v.x + v.y * v.y;
-            if (KIND == K_FWD && Iout) Iout[((y + N / 2) % N) * N + (x + N / 2) % N] = I;
-            const float m = MT[x * N + y];
-            const float d = m - sqrtf(I);
-            err += (double)(d * d);
-            if (KIND == K_FWD) return v;
-            const float f = 1.0f - m / sqrtf(I + 1e-30f);
-            return cmulc(make_float2(v.x * f, v.y * f), pT[x * N + y]);
-        });
+        E.fwd(NoOp(), staged<16>(
+                          [&](int y, int x) {
+                              const float m = MT[x * N + y];
+                              return KIND == K_FWD ? make_float3(m, 0.f, 0.f) : make_float3(m, pT[x * N + y].x, pT[x * N + y].y);
+                          },
+                          [&](int y, int x, float2 v, float3 o) {
+                              const float I = v.x * v.x + v.y * v.y;
+                              if (KIND == K_FWD && Iout) Iout[((y + N / 2) % N) * N + (x + N / 2) % N] = I;
+                              const float m = o.x;
+                              // hardware square root and reciprocal square root (1 ulp)
+                              const float d = m - __builtin_amdgcn_sqrtf(I);
+                              err += (double)(d * d);
+                              if (KIND == K_FWD) return v;
+                              const float f = 1.0f - m * __builtin_amdgcn_rsqf(I + 1e-30f);
+                              return cmulc(make_float2(v.x * f, v.y * f), make_float2(o.y, o.z));
+                          }));
     }
     err = block_sum<NT>(err, red);
     if (t == 0) A.err_pos[k] = err;
@@ -182,7 +198,7 @@ __device__ void process_position(const PosArgs& A, int k, int slot, float2* s, f
         if constexpr (KIND == K_PLAIN) {
             float* V = A.V;
             const float s1 = A.s1;
-            fft2d_inv<N, NT>(s, tw, NoOp(), [&](int y, int x, float2 v) {
+            E.inv(NoOp(), [&](int y, int x, float2 v) {
                 const int i = y * N + x;
                 const float2 b = cscale(v, inv_n2);
                 const float2 tc = cconj(trans_from_volume(V, N2, z0, z1, i, A.dx, A.sigma));
@@ -194,28 +210,32 @@ __device__ void process_position(const PosArgs& A, int k, int slot, float2* s, f
         } else {
             const float2* ta = A.trans + (size_t)a * N2;
             float* g = A.gslab + ((size_t)slot * L + a) * N2;
-            fft2d_inv<N, NT>(s, tw, NoOp(), [&](int y, int x, float2 v) {
-                const int i = y * N + x;
-                const float2 b = cscale(v, inv_n2);
-                const float2 tc = cconj(ta[i]);
-                const float2 q = cmul(tc, cmulc(b, ws[i]));
-                g[i] += -q.y;
-                return cmul(b, tc);
-            });
+            E.inv(NoOp(), staged<8>(
+                              [&](int y, int x) {
+                                  const int i = y * N + x;
+                                  return AdjOp{ta[i], ws[i], g[i]};
+                              },
+                              [&](int y, int x, float2 v, const AdjOp& o) {
+                                  const float2 b = cscale(v, inv_n2);
+                                  const float2 tc = cconj(o.t);
+                                  const float2 q = cmul(tc, cmulc(b, o.w));
+                                  g[y * N + x] = o.g - q.y;
+                                  return cmul(b, tc);
+                              }));
         }
-        if (a > 0) fft2d_fwd<N, NT>(s, tw, NoOp(), [&](int y, int x, float2 v) { return cmulc(v, pT[x * N + y]); });
-        else fft2d_fwd<N, NT>(s, tw);
+        if (a > 0)
+            E.fwd(NoOp(), staged<16>([&](int y, int x) { return pT[x * N + y]; },
+                                     [&](int, int, float2 v, float2 o) { return cmulc(v, o); }));
+        else E.fwd();
     }
-    // s now holds fftn(back_wave_f) (natural-order Fourier domain)
+    // the engine now holds fftn(back_wave_f) (Fourier domain, R state)
 
-    // ---- probe term in the Fourier domain: shift(bw, -d) = ifftn(fftn(bw) .* conj(Phi))
+    // ---- probe term in the Fourier domain: shift(bw, -d) = ifftn(fftn(bw) .* conj(Phi)); slab [x][y]
     if constexpr (KIND == K_BATCH) {
         if (want_probe) {
             float2* ps = A.pslab + (size_t)slot * N2;
-            for (int i = t; i < N2; i += NT) {
-                const int y = i / N, x = i % N;
-                ps[i] = cadd(ps[i], cmulc(s[y * S + x], phase(y, x)));
-            }
+            E.each_R(staged<16>([&](int y, int x) { return ps[x * N + y]; },
+                                [&](int y, int x, float2& v, float2 o) { ps[x * N + y] = cadd(o, cmulc(v, phase(y, x))); }));
         }
     }
 
@@ -225,35 +245,30 @@ __device__ void process_position(const PosArgs& A, int k, int slot, float2* s, f
     if (want_pos) {
         if (exact) {
             // Parseval: sum_x bw conj(-dpsi0/drow) = (1/N^2) sum_q B(q) conj(Psi0(q)) (-2 pi i g/N)
-            for (int i = t; i < N2; i += NT) {
-                const int y = i % N, x = i / N;                // lanes along y: F0 is [x][y]
-                const float2 c = cmulc(s[y * S + x], F0[i]);  // B conj(Psi0)
+            E.each_R(staged<16>([&](int y, int x) { return F0[x * N + y]; }, [&](int y, int x, float2& v, float2 o) {
+                const float2 c = cmulc(v, o);                  // B conj(Psi0)
                 // Re(c * (-i) * 2 pi g / N) = Im(c) * 2 pi g / N
                 dr += (double)c.y * shift_grid<N>(x);
                 dc += (double)c.y * shift_grid<N>(y);
-            }
+            }));
             const double f = 2.0 * M_PI / (double)N / (double)N2;
             dr = block_sum<NT>(dr, red) * f;
             dc = block_sum<NT>(dc, red) * f;
         } else {
             // back_wave_f = ifftn(B); MATLAB gradient() of conj(psi0), one-sided at the edges
-            fft2d_inv<N, NT>(s, tw, NoOp(), [&](int, int, float2 v) { return cscale(v, inv_n2); });
+            E.inv(NoOp(), [&](int, int, float2 v) { return cscale(v, inv_n2); });
             have_bw = true;
             const float2* p0 = save;
-            for (int i = t; i < N2; i += NT) {
-                const int x = i % N, y = i / N;
-                const float2 bw = s[y * S + x];
-                float2 gx, gy;
-                if (x == 0) gx = csub(p0[i + 1], p0[i]);
-                else if (x == N - 1) gx = csub(p0[i], p0[i - 1]);
-                else gx = cscale(csub(p0[i + 1], p0[i - 1]), 0.5f);
-                if (y == 0) gy = csub(p0[i + N], p0[i]);
-                else if (y == N - 1) gy = csub(p0[i], p0[i - N]);
-                else gy = cscale(csub(p0[i + N], p0[i - N]), 0.5f);
+            E.each_C([&](int y, int x, float2& bw) {
+                // clamped neighbours: the edge rows / columns become the one-sided differences
+                const int xp = min(x + 1, N - 1), xm = max(x - 1, 0);
+                const int yp = min(y + 1, N - 1), ym = max(y - 1, 0);
+                const float2 gx = cscale(csub(p0[y * N + xp], p0[y * N + xm]), xp - xm == 2 ? 0.5f : 1.0f);
+                const float2 gy = cscale(csub(p0[yp * N + x], p0[ym * N + x]), yp - ym == 2 ? 0.5f : 1.0f);
                 // Re(conj(g) * bw)
                 dr += (double)(gx.x * bw.x + gx.y * bw.y);
                 dc += (double)(gy.x * bw.x + gy.y * bw.y);
-            }
+            });
             dr = block_sum<NT>(dr, red);
             dc = block_sum<NT>(dc, red);
         }
@@ -276,10 +291,10 @@ __device__ void process_position(const PosArgs& A, int k, int slot, float2* s, f
             };
             if (have_bw) {
                 // MATLAB shifts back_wave_f: fftn(bw) .* conj(Phi) -> ifftn
-                fft2d_fwd<N, NT>(s, tw, NoOp(), [&](int y, int x, float2 v) { return cmulc(v, phase(y, x)); });
-                fft2d_inv<N, NT>(s, tw, NoOp(), upd);
+                E.fwd(NoOp(), [&](int y, int x, float2 v) { return cmulc(v, phase(y, x)); });
+                E.inv(NoOp(), upd);
             } else {
-                fft2d_inv<N, NT>(s, tw, [&](int y, int x, float2 v) { return cmulc(v, phase(y, x)); }, upd);
+                E.inv([&](int y, int x, float2 v) { return cmulc(v, phase(y, x)); }, upd);
             }
         }
         if (t == 0 && want_pos) {
@@ -294,27 +309,29 @@ template <int N, int NT> __device__ __forceinline__ void load_twiddles(float2* t
     __syncthreads();
 }
 
-#define MSET_LDS(N, NT)                                 \
-    __shared__ float2 s[N * Pitch<N>::S];               \
-    __shared__ float2 tw[N], phx[N], phy[N];            \
-    __shared__ double red[(NT + 63) / 64];
+#define MSET_LDS(N, NT)                                          \
+    using Eng = typename Cfg<N>::Eng;                            \
+    __shared__ float2 s[Eng::LDS_ELEMS];                         \
+    __shared__ float2 tw[N], phx[N], phy[N];                     \
+    __shared__ double red[(NT + 63) / 64];                       \
+    load_twiddles<N, NT>(tw, A.tw);                              \
+    Eng E;                                                       \
+    E.init(s, tw);
 
 template <int N>
 __global__ __launch_bounds__(Cfg<N>::NT) void k_plain(PosArgs A, int k) {
     constexpr int NT = Cfg<N>::NT;
     MSET_LDS(N, NT)
-    load_twiddles<N, NT>(tw, A.tw);
-    process_position<N, NT, K_PLAIN>(A, k, 0, s, tw, phx, phy, red);
+    process_position<N, NT, K_PLAIN>(A, k, 0, E, phx, phy, red);
 }
 
 template <int N>
 __global__ __launch_bounds__(Cfg<N>::NT) void k_batch(PosArgs A) {
     constexpr int NT = Cfg<N>::NT;
     MSET_LDS(N, NT)
-    load_twiddles<N, NT>(tw, A.tw);
     for (int p = blockIdx.x; p < A.count; p += gridDim.x) {
         const int k = A.order ? A.order[A.first + p] : A.first + p;
-        process_position<N, NT, K_BATCH>(A, k, blockIdx.x, s, tw, phx, phy, red);
+        process_position<N, NT, K_BATCH>(A, k, blockIdx.x, E, phx, phy, red);
         __syncthreads();
     }
 }
@@ -323,40 +340,40 @@ template <int N>
 __global__ __launch_bounds__(Cfg<N>::NT) void k_fwd(PosArgs A) {
     constexpr int NT = Cfg<N>::NT;
     MSET_LDS(N, NT)
-    load_twiddles<N, NT>(tw, A.tw);
     for (int p = blockIdx.x; p < A.count; p += gridDim.x) {
         const int k = A.order ? A.order[A.first + p] : A.first + p;
-        process_position<N, NT, K_FWD>(A, k, blockIdx.x, s, tw, phx, phy, red);
+        process_position<N, NT, K_FWD>(A, k, blockIdx.x, E, phx, phy, red);
         __syncthreads();
     }
 }
 
-// One plane, row-major in and out unless noted:
-//   dir 0: out^T = fftn(in) written transposed ([x][y]), alpha/beta ignored;
-//   dir 1: out = beta * out + alpha * ifftn(in) (with 1/N^2; beta = 0: plain store).
+// One plane:
+//   dir 0: out = fftn(in), in row-major [y][x], out written transposed ([x][y]); alpha/beta ignored;
+//   dir 1: out = beta * out + alpha * ifftn(in) (with 1/N^2), in transposed ([x][y]), out row-major.
 template <int N>
 __global__ __launch_bounds__(Cfg<N>::NT) void k_plane_fft(const float2* in, float2* out, const float2* twg, int dir,
                                                          float alpha, float beta) {
     constexpr int NT = Cfg<N>::NT;
-    constexpr int S = Pitch<N>::S;
-    __shared__ float2 s[N * S];
+    using Eng = typename Cfg<N>::Eng;
+    __shared__ float2 s[Eng::LDS_ELEMS];
     __shared__ float2 tw[N];
     load_twiddles<N, NT>(tw, twg);
-    for (int i = threadIdx.x; i < N * N; i += NT) s[(i / N) * S + i % N] = in[i];
-    __syncthreads();
+    Eng E;
+    E.init(s, tw);
     if (dir == 0) {
-        fft2d_fwd<N, NT>(s, tw);
-        for (int i = threadIdx.x; i < N * N; i += NT) {
-            const int y = i % N, x = i / N;                    // lanes along y: coalesced transposed store
-            out[i] = s[y * S + x];
-        }
+        E.template fwd<true>([&](int y, int x, float2) { return in[y * N + x]; },
+                             [&](int y, int x, float2 v) {
+                                 out[x * N + y] = v;
+                                 return v;
+                             });
     } else {
-        fft2d_inv<N, NT>(s, tw);
         const float sc = alpha / (float)(N * N);
-        for (int i = threadIdx.x; i < N * N; i += NT) {
-            const float2 w = cscale(s[(i / N) * S + i % N], sc);
-            out[i] = beta == 0.f ? w : cadd(cscale(out[i], beta), w);
-        }
+        E.template inv<true>([&](int y, int x, float2) { return in[x * N + y]; },
+                             [&](int y, int x, float2 v) {
+                                 const float2 w = cscale(v, sc);
+                                 out[y * N + x] = beta == 0.f ? w : cadd(cscale(out[y * N + x], beta), w);
+                                 return v;
+                             });
     }
 }
 
