@@ -273,7 +273,9 @@ int read_errors(mset_ctx* c, int first, int count, double* out) {
 
 // one launch per position, sequential updates (issue only; no host synchronisation)
 int issue_plain(mset_ctx* c) {
+    CHK(ensure_fixed_buffers(c));
     CHK(c->ws->save.ensure((size_t)(c->L + 2) * plane(c), "save waves"));
+    CHK(c->ws->gslab.ensure((size_t)c->L * plane(c), "gradient planes"));
     PosArgs A = base_args(c);
     for (int k = 0; k < c->npos; ++k) {
         Timed t(c, "position_plain");

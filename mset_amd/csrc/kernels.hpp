@@ -19,7 +19,7 @@ struct PosArgs {
     const float2* prop;          // propagator, unshifted
     const float* sqrtM;          // sqrt of the measured patterns, unshifted, [k][y][x]
     float* V;                    // K_PLAIN: volume [z][y][x], updated in place
-    const float2* trans;         // K_BATCH / K_FWD: transmission planes [a][y][x]
+    float2* trans;               // transmission planes [a][y][x] (K_PLAIN computes them itself)
     float2* probe;               // K_PLAIN: probe, updated in place
     const float2* probeF;        // K_BATCH / K_FWD: fftn(probe)
     float* xlist;                // scan positions (MATLAB 1-based pixels); K_PLAIN updates them
@@ -29,7 +29,7 @@ struct PosArgs {
     double* err_pos;             // per position: sum over pixels of (sqrt M - sqrt I)^2
     float* inten;                // K_FWD, optional: |Psi|^2 per position, DC-centred (MATLAB layout)
     float2* save;                // per-slot scratch: (L + 2) planes
-    float* gslab;                // K_BATCH: per-slot object-gradient slab, L planes
+    float* gslab;                // per-slot object-gradient slab, L planes (K_PLAIN: slot 0, this position)
     float2* pslab;               // K_BATCH: per-slot probe-gradient plane (Fourier domain)
     float* dpos;                 // K_BATCH: per-position update terms (2 per position)
 };

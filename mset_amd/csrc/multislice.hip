@@ -8,8 +8,8 @@
 //
 // Three kinds of launch share it:
 //   K_PLAIN  one position per launch, sequential semantics of src/Run_MSET_recon.m:134-158 and
-//            src/CUDA/mset.cu:107-333: the transmission is recomputed from V, and V, the probe and
-//            the position are updated in place before the next launch.
+//            src/CUDA/mset.cu:107-333: the transmission planes are recomputed from V at the start, and
+//            V, the probe and the position are updated in place at the end, before the next launch.
 //   K_BATCH  many positions per launch at a fixed (V, probe): a persistent grid of workgroups, each
 //            accumulating its positions' object gradient into a private slab (L planes) and its
 //            probe-gradient term into a private Fourier-domain plane; slabs are summed afterwards in
@@ -66,10 +66,12 @@ template <int N> __device__ __forceinline__ double shift_grid(int j) {
 
 // transmission of plane a at pixel i from the volume: exp(1i*sigma*(sum_{bin a} V)*dx)
 // (STEP02_FORWARD_4D.m:12-20; same rounding order: sum, x dx, x sigma)
-__device__ __forceinline__ float2 trans_from_volume(const float* __restrict__ V, int n2, int z0, int z1, int i,
-                                                    float dx, float sigma) {
+__device__ __forceinline__ float volume_sum(const float* __restrict__ V, int n2, int z0, int z1, int i) {
     float s = 0.f;
     for (int z = z0; z < z1; ++z) s += V[(size_t)z * n2 + i];
+    return s;
+}
+__device__ __forceinline__ float2 trans_from_sum(float s, float dx, float sigma) {
     const float ph = sigma * (s * dx);
     float sn, cs;
     sincosf(ph, &sn, &cs);
@@ -82,8 +84,9 @@ struct AdjOp {
     float g;
 };
 
+
 template <int N, int NT, int KIND, class Eng>
-__device__ void process_position(const PosArgs& A, int k, int slot, Eng& E, float2* phx, float2* phy, double* red) {
+__device__ __forceinline__ void process_position(const PosArgs& A, int k, int slot, Eng& E, float2* phx, float2* phy, double* red) {
     constexpr int N2 = N * N;
     const int t = threadIdx.x;
     const int L = A.L;
@@ -108,6 +111,48 @@ __device__ void process_position(const PosArgs& A, int k, int slot, Eng& E, floa
     }
     __syncthreads();
     auto phase = [&](int y, int x) { return cmul(phy[y], phx[x]); };
+
+    // ---- sequential kernel: transmission planes from the current volume (the batched kernels get them
+    // from a separate launch); this position's updates touch each bin only after its plane is done, so
+    // the planes stay valid for the whole position
+    if constexpr (KIND == K_PLAIN) {
+        // four pixels per float4, U chunks in flight per thread
+        constexpr int C = N2 / 4, U = 8;
+        const float4* V4 = reinterpret_cast<const float4*>(A.V);
+        for (int a = 0; a < L; ++a) {
+            const int z0 = a * A.bin, z1 = min((a + 1) * A.bin, A.nz);
+            float4* tr = reinterpret_cast<float4*>(A.trans + (size_t)a * N2);
+            for (int c0 = t; c0 < C; c0 += NT * U) {
+                float4 acc[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) acc[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+                for (int z = z0; z < z1; ++z) {
+                    float4 w[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+                        if (c0 + u * NT < C) w[u] = V4[(size_t)z * C + c0 + u * NT];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        acc[u].x += w[u].x;
+                        acc[u].y += w[u].y;
+                        acc[u].z += w[u].z;
+                        acc[u].w += w[u].w;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int c = c0 + u * NT;
+                    if (c < C) {
+                        const float2 t0 = trans_from_sum(acc[u].x, A.dx, A.sigma), t1 = trans_from_sum(acc[u].y, A.dx, A.sigma);
+                        const float2 t2 = trans_from_sum(acc[u].z, A.dx, A.sigma), t3 = trans_from_sum(acc[u].w, A.dx, A.sigma);
+                        tr[2 * c] = make_float4(t0.x, t0.y, t1.x, t1.y);
+                        tr[2 * c + 1] = make_float4(t2.x, t2.y, t3.x, t3.y);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
 
     // ---- incident wave psi0 = ifftn(fftn(probe) .* Phi)
     if constexpr (KIND == K_PLAIN) {
@@ -142,21 +187,11 @@ __device__ void process_position(const PosArgs& A, int k, int slot, Eng& E, floa
 
     // ---- forward sweep: psi_a = ifftn(fftn(psi_{a-1} .* t_a) .* p)
     for (int a = 0; a < L; ++a) {
-        const int z0 = a * A.bin, z1 = min((a + 1) * A.bin, A.nz);
-        if constexpr (KIND == K_PLAIN) {
-            const float* V = A.V;
-            E.fwd([&](int y, int x, float2 v) {
-                                 return cmul(v, trans_from_volume(V, N2, z0, z1, y * N + x, A.dx, A.sigma));
-                             },
-                             staged<16>([&](int y, int x) { return pT[x * N + y]; },
-                                        [&](int, int, float2 v, float2 o) { return cmul(v, o); }));
-        } else {
-            const float2* ta = A.trans + (size_t)a * N2;
-            E.fwd(staged<16>([&](int y, int x) { return ta[y * N + x]; },
-                             [&](int, int, float2 v, float2 o) { return cmul(v, o); }),
-                  staged<16>([&](int y, int x) { return pT[x * N + y]; },
-                             [&](int, int, float2 v, float2 o) { return cmul(v, o); }));
-        }
+        const float2* ta = A.trans + (size_t)a * N2;
+        E.fwd(staged<16>([&](int y, int x) { return ta[y * N + x]; },
+                         [&](int, int, float2 v, float2 o) { return cmul(v, o); }),
+              staged<16>([&](int y, int x) { return pT[x * N + y]; },
+                         [&](int, int, float2 v, float2 o) { return cmul(v, o); }));
         float2* sv = save + (size_t)(a + 1) * N2;
         E.inv(NoOp(), [&](int y, int x, float2 v) {
             const float2 w = cscale(v, inv_n2);
@@ -193,34 +228,41 @@ __device__ void process_position(const PosArgs& A, int k, int slot, Eng& E, floa
 
     // ---- adjoint sweep (STEP03_BACK.m:12-24, mset.cu:232-264)
     for (int a = L - 1; a >= 0; --a) {
-        const int z0 = a * A.bin, z1 = min((a + 1) * A.bin, A.nz);
         const float2* ws = save + (size_t)(exact ? a : a + 1) * N2;
+        const float2* ta = A.trans + (size_t)a * N2;
+        float* g = A.gslab + ((size_t)slot * L + a) * N2;
+        auto adjoint = [&](float2 v, float2 t, float2 w, float& gr) {
+            const float2 b = cscale(v, inv_n2);
+            const float2 tc = cconj(t);
+            const float2 q = cmul(tc, cmulc(b, w));
+            gr = -q.y;                                        // Re(1i * conj(t) * conj(psi) * b)
+            return cmul(b, tc);
+        };
         if constexpr (KIND == K_PLAIN) {
-            float* V = A.V;
-            const float s1 = A.s1;
-            E.inv(NoOp(), [&](int y, int x, float2 v) {
-                const int i = y * N + x;
-                const float2 b = cscale(v, inv_n2);
-                const float2 tc = cconj(trans_from_volume(V, N2, z0, z1, i, A.dx, A.sigma));
-                const float2 q = cmul(tc, cmulc(b, ws[i]));
-                const float gr = -q.y;                       // Re(1i * conj(t) * conj(psi) * b)
-                for (int z = z0; z < z1; ++z) V[(size_t)z * N2 + i] += s1 * gr;
-                return cmul(b, tc);
-            });
+            // the gradient plane of this position (applied to the bin after the sweep)
+            E.inv(NoOp(), staged<16>(
+                              [&](int y, int x) {
+                                  const int i = y * N + x;
+                                  return make_float4(ta[i].x, ta[i].y, ws[i].x, ws[i].y);
+                              },
+                              [&](int y, int x, float2 v, float4 o) {
+                                  float gr;
+                                  const float2 r = adjoint(v, make_float2(o.x, o.y), make_float2(o.z, o.w), gr);
+                                  g[y * N + x] = gr;
+                                  return r;
+                              }));
         } else {
-            const float2* ta = A.trans + (size_t)a * N2;
-            float* g = A.gslab + ((size_t)slot * L + a) * N2;
+            // accumulated into this workgroup's slab
             E.inv(NoOp(), staged<8>(
                               [&](int y, int x) {
                                   const int i = y * N + x;
                                   return AdjOp{ta[i], ws[i], g[i]};
                               },
                               [&](int y, int x, float2 v, const AdjOp& o) {
-                                  const float2 b = cscale(v, inv_n2);
-                                  const float2 tc = cconj(o.t);
-                                  const float2 q = cmul(tc, cmulc(b, o.w));
-                                  g[y * N + x] = o.g - q.y;
-                                  return cmul(b, tc);
+                                  float gr;
+                                  const float2 r = adjoint(v, o.t, o.w, gr);
+                                  g[y * N + x] = o.g + gr;
+                                  return r;
                               }));
         }
         if (a > 0)
@@ -300,6 +342,30 @@ __device__ void process_position(const PosArgs& A, int k, int slot, Eng& E, floa
         if (t == 0 && want_pos) {
             A.xlist[k] = (float)(row + (double)A.s3 * dr);
             A.ylist[k] = (float)(col + (double)A.s3 * dc);
+        }
+        // object step: every voxel of bin a moves by s1 * (gradient plane a) (STEP03_BACK.m:84)
+        __syncthreads();
+        constexpr int C = N2 / 4, U = 8;
+        const float4* g4 = reinterpret_cast<const float4*>(A.gslab);
+        float4* V4 = reinterpret_cast<float4*>(A.V);
+        const float s1 = A.s1;
+        for (int z = 0; z < A.nz; ++z) {
+            const float4* gz = g4 + (size_t)(z / A.bin) * C;
+            float4* vz = V4 + (size_t)z * C;
+            for (int c0 = t; c0 < C; c0 += NT * U) {
+                float4 gv[U], vv[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (c0 + u * NT < C) {
+                        gv[u] = gz[c0 + u * NT];
+                        vv[u] = vz[c0 + u * NT];
+                    }
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (c0 + u * NT < C)
+                        vz[c0 + u * NT] = make_float4(vv[u].x + s1 * gv[u].x, vv[u].y + s1 * gv[u].y,
+                                                      vv[u].z + s1 * gv[u].z, vv[u].w + s1 * gv[u].w);
+            }
         }
     }
 }
