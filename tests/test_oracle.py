@@ -58,6 +58,23 @@ def test_constants_match_saved_state(cuau):
     assert abs(sig / man['sigma'] - 1) < 1e-14          # interaction_constant.m
 
 
+def test_fixture_probe_generation(cuau):
+    """Func_generate_probe_wave.m:14-56 with C10 = -probeDefocus reproduces the saved probe up to the
+    real positive factor of the normalisation (STEP00_INIT.m:214-218, pinned separately by Parseval)."""
+    man, s = cuau['man'], cuau['snap']
+    lam, _, dx = _phys(man)
+    p = O.probe(24, dx, lam, man['alpha_mrad'], C10=-man['probeDefocus_A'])
+    for j in range(s['probe_wfn'].shape[0]):
+        q = s['probe_wfn'][j].astype(np.complex128)
+        a = np.vdot(p, q) / np.vdot(p, p)
+        assert abs(a.imag) < 1e-6 * abs(a) and a.real > 0
+        assert rel(a * p, q) < 1e-6
+    # the defocus sign matters: the opposite sign is far off
+    pw = O.probe(24, dx, lam, man['alpha_mrad'], C10=man['probeDefocus_A'])
+    a = np.vdot(pw, s['probe_wfn'][0]) / np.vdot(pw, pw)
+    assert rel(a * pw, s['probe_wfn'][0]) > 0.5
+
+
 def test_fixture1_propagator(cuau):
     man, s = cuau['man'], cuau['snap']
     lam, _, dx = _phys(man)
