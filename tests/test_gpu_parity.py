@@ -100,7 +100,8 @@ def test_rotation_vs_oracle(mset, shape, ang):
 # ------------------------------------------------------------------ operator parity on synthetic states
 @pytest.mark.parametrize('n,nz,binning,mode', [(24, 6, 2, O.MODE_REF), (32, 12, 3, O.MODE_EXACT),
                                                (64, 8, 4, O.MODE_REF), (128, 16, 4, O.MODE_EXACT),
-                                               (48, 5, 2, O.MODE_EXACT), (96, 4, 1, O.MODE_REF)])
+                                               (128, 7, 2, O.MODE_REF), (48, 5, 2, O.MODE_EXACT),
+                                               (96, 4, 1, O.MODE_REF)])
 def test_grad_at_fixed_state_vs_oracle(mset, n, nz, binning, mode):
     c = small_case(n=n, nz=nz, binning=binning, npos=5, seed=n + nz, spread=n / 8)
     print('case: grad', n, nz, binning, mode, flush=True)
@@ -121,7 +122,8 @@ def test_grad_at_fixed_state_vs_oracle(mset, n, nz, binning, mode):
 
 
 @pytest.mark.parametrize('n,nz,binning,mode', [(24, 6, 2, O.MODE_REF), (32, 9, 3, O.MODE_EXACT),
-                                               (64, 8, 4, O.MODE_REF)])
+                                               (64, 8, 4, O.MODE_REF), (48, 7, 3, O.MODE_EXACT),
+                                               (128, 8, 4, O.MODE_REF), (128, 5, 2, O.MODE_EXACT)])
 def test_sequential_run_vs_oracle(mset, n, nz, binning, mode):
     """'run_mset' semantics: per-position updates of V, probe and position (Run_MSET_recon.m:134-158)."""
     c = small_case(n=n, nz=nz, binning=binning, npos=6, seed=3 * n + nz, spread=n / 8)
