@@ -32,10 +32,13 @@ extern "C" {
 #define MSET_EINVAL 1          /* bad argument, shape or call order */
 #define MSET_ENOMEM 2          /* device allocation failed */
 #define MSET_EDEVICE 3         /* HIP runtime / kernel failure */
-#define MSET_EUNSUPPORTED 4    /* valid for the reference but not provided here (e.g. the SSET method) */
+#define MSET_EUNSUPPORTED 4    /* valid for the reference but not provided here (e.g. a plane size not built) */
 
-#define MSET_METHOD_MSET 1     /* 'run_mset' / 'error_mset' */
-#define MSET_METHOD_SSET 2     /* 'run_sset' / 'error_sset' (src/CUDA/mset.cu:345-546): not provided */
+#define MSET_METHOD_MSET 1     /* 'run_mset' / 'error_mset': multislice, STEP02_FORWARD_4D.m / STEP03_BACK.m */
+#define MSET_METHOD_SSET 2     /* 'run_sset' / 'error_sset': single slice, STEP02_FORWARD_4D_SSET.m /
+                                  STEP03_BACK_SSET.m (src/CUDA/mset.cu:345-546, 700-810): the whole depth
+                                  projected into one transmission, no propagation, object term paired with
+                                  the incident wave, the same probe and position updates */
 
 #define MSET_GRAD_REF 0        /* STEP03_BACK.m:14-15 pairing: conj of the wave after plane a */
 #define MSET_GRAD_EXACT 1      /* exact adjoint: wave entering plane a; spectral position derivative */
@@ -69,13 +72,13 @@ void* mset_stream(mset_ctx* ctx);
 int mset_upload(mset_ctx* ctx, const mset_params* p, const float* RVol, const float* init_wave2D,
                 const float* prop2D, const float* scan_xlist, const float* scan_ylist, const float* measured);
 
-/* 'run_mset' (mex_main.cu:98-155): one sequential pass over the positions with the per-position
+/* 'run_mset' / 'run_sset' (mex_main.cu:98-155): one sequential pass over the positions with the per-position
  * updates of Run_MSET_recon.m:134-158; returns the updated RVol, probe, scan lists and, if error_out
  * is not NULL, the error (sum over positions of mean((sqrt M - sqrt I)^2) / (N_scan_x N_scan_y)). */
 int mset_run(mset_ctx* ctx, int method, float* RVol_out, float* probe_out, float* scan_xlist_out,
              float* scan_ylist_out, float* error_out);
 
-/* 'error_mset' (mex_main.cu:157-184): forward + loss over every position, no updates. */
+/* 'error_mset' / 'error_sset' (mex_main.cu:157-184): forward + loss over every position, no updates. */
 int mset_error(mset_ctx* ctx, int method, float* error_out);
 
 /* ---- device-resident entry points (no host copies of the state) ------------------------------- */
@@ -98,6 +101,7 @@ int mset_set_state(mset_ctx* ctx, const float* RVol, const float* probe, const f
  *   dpos   float[count][2]: per-position (row, col) update terms (position += step(3) * dpos);
  *   error  sum of the positions' error terms, normalised as in mset_run. */
 int mset_grad(mset_ctx* ctx, int first, int count, float* dV, float* dprobe, float* dpos, double* error_out);
+/* (mset_grad, mset_waves, mset_simulate and the mset_recon_* tilt series use the multislice operator.) */
 
 /* The incident wave and the wave after each plane for position k at the resident state, i.e.
  * tmp_probe_wave and save_wave_f of STEP02_FORWARD_4D.m:27-35: interleaved complex [L + 1][ny][nx]. */

@@ -97,6 +97,7 @@ struct mset_ctx {
     int cus = 0;
     mset_params p{};
     int n = 0, nz = 0, L = 0, npos = 0;
+    int method = MSET_METHOD_MSET;   // operator of the current call (MSET, or SSET = one plane of depth nz)
     bool uploaded = false;
     DevBuf<float> V, sqrtM, x, y, dpos, Mtmp;
     DevBuf<float2> probe, prop, tw;
@@ -171,10 +172,11 @@ int check_uploaded(mset_ctx* c) {
     return MSET_OK;
 }
 
-int check_method(int method) {
-    if (method == MSET_METHOD_SSET)
-        return fail(MSET_EUNSUPPORTED, "the SSET method (run_sset/error_sset) is not provided by this engine");
-    if (method != MSET_METHOD_MSET) return fail(MSET_EINVAL, "method must be MSET_METHOD_MSET");
+// selects the operator of the following work on the context: 'run_mset' / 'run_sset' (mex_main.cu:26-35)
+int set_method(mset_ctx* c, int method) {
+    if (method != MSET_METHOD_MSET && method != MSET_METHOD_SSET)
+        return fail(MSET_EINVAL, "method must be MSET_METHOD_MSET or MSET_METHOD_SSET");
+    c->method = method;
     return MSET_OK;
 }
 
@@ -187,12 +189,18 @@ void twiddles(int n, std::vector<float2>& tw) {
 }
 
 size_t plane(const mset_ctx* c) { return (size_t)c->n * c->n; }
+bool sset(const mset_ctx* c) { return c->method == MSET_METHOD_SSET; }
+// planes and slices per plane of the current operator: SSET projects the whole depth into one plane
+// (STEP02_FORWARD_4D_SSET.m:9)
+int planes(const mset_ctx* c) { return sset(c) ? 1 : c->L; }
+int binning(const mset_ctx* c) { return sset(c) ? c->nz : c->p.slice_binning; }
 
 PosArgs base_args(mset_ctx* c) {
     PosArgs A{};
     A.nz = c->nz;
-    A.L = c->L;
-    A.bin = c->p.slice_binning;
+    A.L = planes(c);
+    A.bin = binning(c);
+    A.sset = sset(c) ? 1 : 0;
     A.sigma = c->p.sigma;
     A.dx = c->p.potential_pixelsize;
     A.cent = (float)std::floor((c->n + 1) / 2.0 + 0.5);
@@ -251,7 +259,7 @@ int ensure_batch_buffers(mset_ctx* c) {
 // transmission planes + fftn(probe) for the fixed-state kernels
 int prepare_fixed(mset_ctx* c) {
     CHK(ensure_fixed_buffers(c));
-    HIPCHK(launch_transmission(c->V.p, c->ws->trans.p, c->n, c->nz, c->p.slice_binning, c->p.sigma,
+    HIPCHK(launch_transmission(c->V.p, c->ws->trans.p, c->n, c->nz, binning(c), sset(c) ? 1 : 0, c->p.sigma,
                                c->p.potential_pixelsize, c->stream));
     HIPCHK(launch_plane_fft(c->n, c->probe.p, c->ws->probeF.p, c->tw.p, 0, 1.f, 0.f, c->stream));
     return MSET_OK;
@@ -307,13 +315,13 @@ int issue_grad(mset_ctx* c, int first, int count, bool apply, int want) {
     A.first = first;
     A.count = count;
     A.want = want;
-    HIPCHK(launch_fill(W.gslab.p, sizeof(float) * (size_t)grid * c->L * plane(c), c->stream));
+    HIPCHK(launch_fill(W.gslab.p, sizeof(float) * (size_t)grid * planes(c) * plane(c), c->stream));
     if (want & WANT_PROBE) HIPCHK(launch_fill(W.pslab.p, sizeof(float2) * (size_t)grid * plane(c), c->stream));
     {
         Timed t(c, "position_batch");
         HIPCHK(launch_batch(c->n, A, grid, c->stream));
     }
-    HIPCHK(launch_reduce_slabs(W.gslab.p, grid, (size_t)c->L * plane(c), W.G.p, c->stream));
+    HIPCHK(launch_reduce_slabs(W.gslab.p, grid, (size_t)planes(c) * plane(c), W.G.p, c->stream));
     if (want & WANT_PROBE) {
         // in place into slab 0 (each element is read and written by the same thread), then PG = ifftn
         HIPCHK(launch_reduce_cslabs(W.pslab.p, grid, plane(c), W.pslab.p, c->stream));
@@ -321,7 +329,7 @@ int issue_grad(mset_ctx* c, int first, int count, bool apply, int want) {
     }
     if (apply) {
         if (c->p.step_size[0] != 0.f)
-            HIPCHK(launch_update_volume(c->V.p, W.G.p, c->n, c->nz, c->p.slice_binning, c->p.step_size[0], c->stream));
+            HIPCHK(launch_update_volume(c->V.p, W.G.p, c->n, c->nz, binning(c), c->p.step_size[0], c->stream));
         if (c->p.step_size[1] != 0.f && (want & WANT_PROBE))
             HIPCHK(launch_plane_fft(c->n, W.pslab.p, c->probe.p, c->tw.p, 1, -c->p.step_size[1], 1.f, c->stream));
         if (c->p.step_size[2] != 0.f && (want & WANT_POS))
@@ -543,7 +551,7 @@ int mset_upload(mset_ctx* c, const mset_params* p, const float* RVol, const floa
 
 int mset_run_resident(mset_ctx* c, int method, double* err_out) {
     CHK(check_uploaded(c));
-    CHK(check_method(method));
+    CHK(set_method(c, method));
     CHK(issue_plain(c));
     if (err_out) return read_errors(c, 0, c->npos, err_out);
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -552,7 +560,7 @@ int mset_run_resident(mset_ctx* c, int method, double* err_out) {
 
 int mset_error_resident(mset_ctx* c, int method, double* err_out) {
     CHK(check_uploaded(c));
-    CHK(check_method(method));
+    CHK(set_method(c, method));
     if (!err_out) return fail(MSET_EINVAL, "null error output");
     CHK(issue_fwd(c, nullptr));
     return read_errors(c, 0, c->npos, err_out);
@@ -560,6 +568,7 @@ int mset_error_resident(mset_ctx* c, int method, double* err_out) {
 
 int mset_simulate(mset_ctx* c, float* intensity_out) {
     CHK(check_uploaded(c));
+    CHK(set_method(c, MSET_METHOD_MSET));
     if (!intensity_out) return fail(MSET_EINVAL, "null output");
     DevBuf<float> I;
     CHK(I.ensure(plane(c) * c->npos, "intensities"));
@@ -611,6 +620,7 @@ int mset_error(mset_ctx* c, int method, float* err_out) {
 int mset_grad(mset_ctx* c, int first, int count, float* dV, float* dprobe, float* dpos, double* err_out) {
     CHK(check_uploaded(c));
     if (first < 0 || count <= 0 || first + count > c->npos) return fail(MSET_EINVAL, "position range out of bounds");
+    CHK(set_method(c, MSET_METHOD_MSET));
     CHK(issue_grad(c, first, count, false, WANT_PROBE | WANT_POS));
     const size_t n2 = plane(c);
     if (dV) HIPCHK(hipMemcpyAsync(dV, c->ws->G.p, sizeof(float) * n2 * c->L, hipMemcpyDeviceToHost, c->stream));
@@ -625,6 +635,7 @@ int mset_grad(mset_ctx* c, int first, int count, float* dV, float* dprobe, float
 int mset_waves(mset_ctx* c, int k, float* waves_out) {
     CHK(check_uploaded(c));
     if (k < 0 || k >= c->npos || !waves_out) return fail(MSET_EINVAL, "bad position index or null output");
+    CHK(set_method(c, MSET_METHOD_MSET));
     CHK(ensure_batch_buffers(c));
     CHK(prepare_fixed(c));
     PosArgs A = base_args(c);
@@ -641,7 +652,7 @@ int mset_waves(mset_ctx* c, int k, float* waves_out) {
 
 int mset_run_batched(mset_ctx* c, int method, int batch, double* err_out) {
     CHK(check_uploaded(c));
-    CHK(check_method(method));
+    CHK(set_method(c, method));
     if (batch <= 0) return fail(MSET_EINVAL, "batch must be positive");
     CHK(issue_batched(c, batch));
     if (err_out) return read_errors(c, 0, c->npos, err_out);

@@ -28,15 +28,16 @@ __global__ void k_ingest(const float* __restrict__ M, float* __restrict__ out, i
 }
 
 // t_a = exp(1i * sigma * (sum_{z in bin a} V) * dx)  (STEP02_FORWARD_4D.m:12-20)
-__global__ void k_trans(const float* __restrict__ V, float2* __restrict__ T, int n2, int nz, int bin, int L, float sigma,
-                        float dx) {
+__global__ void k_trans(const float* __restrict__ V, float2* __restrict__ T, int n2, int nz, int bin, int L, int sset,
+                        float sigma, float dx) {
     const long long tot = (long long)n2 * L;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < tot; i += (long long)gridDim.x * blockDim.x) {
         const int a = (int)(i / n2), p = (int)(i - (long long)a * n2);
         const int z0 = a * bin, z1 = min(z0 + bin, nz);
         float s = 0.f;
         for (int z = z0; z < z1; ++z) s += V[(size_t)z * n2 + p];
-        const float ph = sigma * (s * dx);
+        // STEP02_FORWARD_4D.m:18-19 rounds sigma * (sum * dx); STEP02_FORWARD_4D_SSET.m:9 (sigma * sum) * dx
+        const float ph = sset ? (sigma * s) * dx : sigma * (s * dx);
         float sn, cs;
         sincosf(ph, &sn, &cs);
         T[i] = make_float2(cs, sn);
@@ -97,9 +98,10 @@ hipError_t launch_ingest_patterns(const float* M, float* sqrtM, int n, long long
     return hipGetLastError();
 }
 
-hipError_t launch_transmission(const float* V, float2* T, int n, int nz, int bin, float sigma, float dx, hipStream_t st) {
+hipError_t launch_transmission(const float* V, float2* T, int n, int nz, int bin, int sset, float sigma, float dx,
+                               hipStream_t st) {
     const int L = (nz + bin - 1) / bin;
-    hipLaunchKernelGGL(k_trans, grid_for((size_t)n * n * L), dim3(256), 0, st, V, T, n * n, nz, bin, L, sigma, dx);
+    hipLaunchKernelGGL(k_trans, grid_for((size_t)n * n * L), dim3(256), 0, st, V, T, n * n, nz, bin, L, sset, sigma, dx);
     return hipGetLastError();
 }
 

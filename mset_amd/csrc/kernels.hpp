@@ -13,7 +13,8 @@ struct PosArgs {
     float sigma, dx;             // interaction constant, voxel size: phase = sigma * (dx * sum V)
     float cent;                  // round((N+1)/2), the 1-based centre of the scan coordinates
     float s1, s2, s3;            // step sizes (object, probe, position)
-    int grad_mode;               // GRAD_REF or GRAD_EXACT
+    int grad_mode;
+    int sset;                    // single-slice operator: L = 1, bin = nz, no propagation, phase (sigma sum V) dx               // GRAD_REF or GRAD_EXACT
     int want;                    // WANT_PROBE | WANT_POS
     const float2* tw;            // W_N^k, k < N
     const float2* prop;          // propagator, unshifted
@@ -45,7 +46,7 @@ hipError_t occupancy_batch(int n, int* blocks_per_cu);
 
 // element-wise kernels (elementwise.hip)
 hipError_t launch_ingest_patterns(const float* M, float* sqrtM, int n, long long npos, hipStream_t st);
-hipError_t launch_transmission(const float* V, float2* trans, int n, int nz, int bin, float sigma, float dx,
+hipError_t launch_transmission(const float* V, float2* trans, int n, int nz, int bin, int sset, float sigma, float dx,
                                hipStream_t st);
 hipError_t launch_fill(void* p, size_t bytes, hipStream_t st);
 hipError_t launch_reduce_slabs(const float* slab, int nslab, size_t n, float* out, hipStream_t st);

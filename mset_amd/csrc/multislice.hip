@@ -71,8 +71,9 @@ __device__ __forceinline__ float volume_sum(const float* __restrict__ V, int n2,
     for (int z = z0; z < z1; ++z) s += V[(size_t)z * n2 + i];
     return s;
 }
-__device__ __forceinline__ float2 trans_from_sum(float s, float dx, float sigma) {
-    const float ph = sigma * (s * dx);
+__device__ __forceinline__ float2 trans_from_sum(float s, float dx, float sigma, int sset) {
+    // STEP02_FORWARD_4D.m:18-19 rounds sigma * (sum * dx); STEP02_FORWARD_4D_SSET.m:9 (sigma * sum) * dx
+    const float ph = sset ? (sigma * s) * dx : sigma * (s * dx);
     float sn, cs;
     sincosf(ph, &sn, &cs);
     return make_float2(cs, sn);
@@ -143,8 +144,8 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
                 for (int u = 0; u < U; ++u) {
                     const int c = c0 + u * NT;
                     if (c < C) {
-                        const float2 t0 = trans_from_sum(acc[u].x, A.dx, A.sigma), t1 = trans_from_sum(acc[u].y, A.dx, A.sigma);
-                        const float2 t2 = trans_from_sum(acc[u].z, A.dx, A.sigma), t3 = trans_from_sum(acc[u].w, A.dx, A.sigma);
+                        const float2 t0 = trans_from_sum(acc[u].x, A.dx, A.sigma, A.sset), t1 = trans_from_sum(acc[u].y, A.dx, A.sigma, A.sset);
+                        const float2 t2 = trans_from_sum(acc[u].z, A.dx, A.sigma, A.sset), t3 = trans_from_sum(acc[u].w, A.dx, A.sigma, A.sset);
                         tr[2 * c] = make_float4(t0.x, t0.y, t1.x, t1.y);
                         tr[2 * c + 1] = make_float4(t2.x, t2.y, t3.x, t3.y);
                     }
@@ -165,8 +166,12 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
                                  return f;
                              });
         E.inv(NoOp(), [&](int y, int x, float2 v) {
-            const float2 w = cscale(v, inv_n2);
+            float2 w = cscale(v, inv_n2);
             save[y * N + x] = w;
+            if (A.sset) {                                   // single slice: exit wave psi0 .* t, no propagation
+                w = cmul(w, A.trans[y * N + x]);
+                save[N2 + y * N + x] = w;
+            }
             return w;
         });
     } else {
@@ -179,14 +184,18 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
                                               return f;
                                           }),
                                [&](int y, int x, float2 v) {
-                                   const float2 w = cscale(v, inv_n2);
+                                   float2 w = cscale(v, inv_n2);
                                    if (KIND == K_BATCH) save[y * N + x] = w;
+                                   if (A.sset) {            // single slice: exit wave psi0 .* t, no propagation
+                                       w = cmul(w, A.trans[y * N + x]);
+                                       if (KIND == K_BATCH) save[N2 + y * N + x] = w;
+                                   }
                                    return w;
                                });
     }
 
     // ---- forward sweep: psi_a = ifftn(fftn(psi_{a-1} .* t_a) .* p)
-    for (int a = 0; a < L; ++a) {
+    for (int a = 0; a < (A.sset ? 0 : L); ++a) {
         const float2* ta = A.trans + (size_t)a * N2;
         E.fwd(staged<16>([&](int y, int x) { return ta[y * N + x]; },
                          [&](int, int, float2 v, float2 o) { return cmul(v, o); }),
@@ -208,7 +217,9 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
         E.fwd(NoOp(), staged<16>(
                           [&](int y, int x) {
                               const float m = MT[x * N + y];
-                              return KIND == K_FWD ? make_float3(m, 0.f, 0.f) : make_float3(m, pT[x * N + y].x, pT[x * N + y].y);
+                              if (KIND == K_FWD) return make_float3(m, 0.f, 0.f);
+                              if (A.sset) return make_float3(m, 1.f, 0.f);     // no back-propagator
+                              return make_float3(m, pT[x * N + y].x, pT[x * N + y].y);
                           },
                           [&](int y, int x, float2 v, float3 o) {
                               const float I = v.x * v.x + v.y * v.y;
@@ -228,7 +239,8 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
 
     // ---- adjoint sweep (STEP03_BACK.m:12-24, mset.cu:232-264)
     for (int a = L - 1; a >= 0; --a) {
-        const float2* ws = save + (size_t)(exact ? a : a + 1) * N2;
+        // the single-slice operator pairs the incident wave (STEP03_BACK_SSET.m:13-14)
+        const float2* ws = save + (size_t)((exact || A.sset) ? a : a + 1) * N2;
         const float2* ta = A.trans + (size_t)a * N2;
         float* g = A.gslab + ((size_t)slot * L + a) * N2;
         auto adjoint = [&](float2 v, float2 t, float2 w, float& gr) {

@@ -12,6 +12,8 @@
 //   Func_generate_probe_wave_scan_sub_pixel.m -> subpixel_shift
 //   STEP02_FORWARD_4D.m:6-51                  -> forward
 //   STEP03_BACK.m:6-39                        -> back
+//   STEP02_FORWARD_4D_SSET.m:9-35             -> forward_sset, transmission_sset
+//   STEP03_BACK_SSET.m:9-28                   -> back_sset
 //   Run_MSET_recon.m:133-158 (device 0)       -> oracle_run_tilt   (sequential per-position updates)
 //   Run_MSET_recon.m:207-222 (device 0)       -> oracle_error_tilt (forward + loss only)
 //   Func_rot_3Dvol_FourierShear.m, Func_inv_rot_3Dvol_FourierShear.m,
@@ -207,6 +209,7 @@ struct Geo {
     int L;          // numPlanes = ceil(nz / bin)
     double sigma;   // interaction constant, rad/(V A)
     double dx;      // potential_pixelsize, A
+    bool sset;      // single-slice operator: one plane holding the whole depth, no propagation
 };
 
 inline int bin_lo(const Geo& g, int a) { return a * g.bin; }
@@ -290,33 +293,13 @@ template <typename T> void matlab_gradient(const cx<T>* f, int n, cx<T>* gx, cx<
         }
 }
 
-enum { MODE_REF = 0, MODE_EXACT = 1 };
+enum { MODE_REF = 0, MODE_EXACT = 1, MODE_SSET = 2 };
 
-// STEP03_BACK.m:9-39.  Writes the complex gradient grad (n2*nz, replicated over each bin) and the final
-// back-propagated wave bw = back_wave_f (real space, MATLAB layout).  Returns the position update terms
-// (drow, dcol) = Re sum(d_x conj(psi0) .* bw), Re sum(d_y conj(psi0) .* bw) without the step factor.
-// mode REF uses save(a) (STEP03_BACK.m:14-15); EXACT uses the wave entering plane a and the spectral
-// derivative of the Fourier-shift for the position terms.
+// Position update terms (STEP03_BACK.m:35-38): (drow, dcol) = Re sum(d_x conj(psi0) .* bw), Re sum(d_y
+// conj(psi0) .* bw); d is MATLAB gradient() (mode REF) or the spectral derivative of the shift (EXACT).
 template <typename T>
-void back(const Geo& g, int mode, const cx<T>* trans, const cx<T>* save, const cx<T>* psi0,
-          const cx<T>* resi, const cx<T>* back_prop, cx<T>* grad, cx<T>* bw, double x1, double y1,
-          double* drow, double* dcol) {
+void position_terms(const Geo& g, int mode, const cx<T>* psi0, const cx<T>* bw, double* drow, double* dcol) {
     const size_t n2 = (size_t)g.n * g.n;
-    std::vector<cx<T>> b(resi, resi + n2);
-    for (int a = g.L - 1; a >= 0; --a) {
-        for (size_t i = 0; i < n2; ++i) b[i] *= back_prop[i];
-        fft2c<T>(b.data(), g.n, +1);
-        const cx<T>* ws = (mode == MODE_REF) ? save + (size_t)a * n2 : (a == 0 ? psi0 : save + (size_t)(a - 1) * n2);
-        cx<T>* g0 = grad + (size_t)bin_lo(g, a) * n2;
-        for (size_t i = 0; i < n2; ++i)
-            g0[i] = cx<T>(0, 1) * std::conj(trans[(size_t)a * n2 + i]) * std::conj(ws[i]) * b[i];
-        for (int z = bin_lo(g, a) + 1; z < bin_hi(g, a); ++z) std::memcpy(grad + (size_t)z * n2, g0, sizeof(cx<T>) * n2);
-        for (size_t i = 0; i < n2; ++i) b[i] *= std::conj(trans[(size_t)a * n2 + i]);
-        fft2c<T>(b.data(), g.n, -1);
-    }
-    fft2c<T>(b.data(), g.n, +1);
-    std::memcpy(bw, b.data(), sizeof(cx<T>) * n2);
-
     const int n = g.n;
     std::vector<cx<T>> cp(n2), dX(n2), dY(n2);
     if (mode == MODE_REF) {
@@ -351,6 +334,35 @@ void back(const Geo& g, int mode, const cx<T>* trans, const cx<T>* save, const c
     if (std::is_same<T, float>::value) { sx = fx; sy = fy; }
     *drow = sx;
     *dcol = sy;
+}
+
+
+// STEP03_BACK.m:9-39.  Writes the complex gradient grad (n2*nz, replicated over each bin) and the final
+// back-propagated wave bw = back_wave_f (real space, MATLAB layout).  Returns the position update terms
+// (drow, dcol) = Re sum(d_x conj(psi0) .* bw), Re sum(d_y conj(psi0) .* bw) without the step factor.
+// mode REF uses save(a) (STEP03_BACK.m:14-15); EXACT uses the wave entering plane a and the spectral
+// derivative of the Fourier-shift for the position terms.
+template <typename T>
+void back(const Geo& g, int mode, const cx<T>* trans, const cx<T>* save, const cx<T>* psi0,
+          const cx<T>* resi, const cx<T>* back_prop, cx<T>* grad, cx<T>* bw, double x1, double y1,
+          double* drow, double* dcol) {
+    const size_t n2 = (size_t)g.n * g.n;
+    std::vector<cx<T>> b(resi, resi + n2);
+    for (int a = g.L - 1; a >= 0; --a) {
+        for (size_t i = 0; i < n2; ++i) b[i] *= back_prop[i];
+        fft2c<T>(b.data(), g.n, +1);
+        const cx<T>* ws = (mode == MODE_REF) ? save + (size_t)a * n2 : (a == 0 ? psi0 : save + (size_t)(a - 1) * n2);
+        cx<T>* g0 = grad + (size_t)bin_lo(g, a) * n2;
+        for (size_t i = 0; i < n2; ++i)
+            g0[i] = cx<T>(0, 1) * std::conj(trans[(size_t)a * n2 + i]) * std::conj(ws[i]) * b[i];
+        for (int z = bin_lo(g, a) + 1; z < bin_hi(g, a); ++z) std::memcpy(grad + (size_t)z * n2, g0, sizeof(cx<T>) * n2);
+        for (size_t i = 0; i < n2; ++i) b[i] *= std::conj(trans[(size_t)a * n2 + i]);
+        fft2c<T>(b.data(), g.n, -1);
+    }
+    fft2c<T>(b.data(), g.n, +1);
+    std::memcpy(bw, b.data(), sizeof(cx<T>) * n2);
+
+    position_terms<T>(g, mode, psi0, bw, drow, dcol);
     (void)x1; (void)y1;
 }
 
@@ -361,6 +373,68 @@ void probe_update(cx<T>* probe, const cx<T>* bw, int n, double x1, double y1, do
     std::vector<cx<T>> sh((size_t)n * n);
     subpixel_shift<T>(bw, sh.data(), n, (double)(T)(c2 - x1), (double)(T)(c2 - y1));
     for (size_t i = 0; i < (size_t)n * n; ++i) probe[i] -= (T)s2 * sh[i];
+}
+
+// STEP02_FORWARD_4D_SSET.m:9-35 : the projected potential of the whole depth as one transmission, no
+// propagation: Psi = fftn(psi0 .* proj_trans); residual and error as STEP02_FORWARD_4D.m.
+template <typename T>
+double forward_sset(const Geo& g, const cx<T>* ptrans, const cx<T>* psi0, const T* M, cx<T>* resi) {
+    const size_t n2 = (size_t)g.n * g.n;
+    std::vector<cx<T>> w(n2);
+    for (size_t i = 0; i < n2; ++i) w[i] = psi0[i] * ptrans[i];
+    fft2c<T>(w.data(), g.n, -1);
+    double err = 0.0;
+    for (size_t i = 0; i < n2; ++i) {
+        const T I = std::norm(w[i]);
+        const T sm = std::sqrt(M[i]);
+        if (resi) resi[i] = w[i] * (T(1) - sm / std::sqrt(I + (T)1e-30));
+        const T d = sm - std::sqrt(I);
+        err += (double)(d * d);
+    }
+    return err / (double)n2;
+}
+
+// STEP03_BACK_SSET.m:9-28 : b = ifftn(resi); grad = 1i conj(proj_trans) conj(psi0) b on every slice;
+// back_wave_f = b .* conj(proj_trans); position terms as STEP03_BACK.m.
+template <typename T>
+void back_sset(const Geo& g, int mode, const cx<T>* ptrans, const cx<T>* psi0, const cx<T>* resi, cx<T>* grad,
+               cx<T>* bw, double* drow, double* dcol) {
+    const size_t n2 = (size_t)g.n * g.n;
+    std::vector<cx<T>> b(resi, resi + n2);
+    fft2c<T>(b.data(), g.n, +1);
+    for (size_t i = 0; i < n2; ++i) grad[i] = cx<T>(0, 1) * std::conj(ptrans[i]) * std::conj(psi0[i]) * b[i];
+    for (int z = 1; z < g.nz; ++z) std::memcpy(grad + (size_t)z * n2, grad, sizeof(cx<T>) * n2);
+    for (size_t i = 0; i < n2; ++i) bw[i] = b[i] * std::conj(ptrans[i]);
+    position_terms<T>(g, mode, psi0, bw, drow, dcol);
+}
+
+// STEP02_FORWARD_4D_SSET.m:9 : proj_trans = exp(1i*sigma*sum(RVol,3)*dx), evaluated left to right
+template <typename T> void transmission_sset(const Geo& g, const T* V, cx<T>* ptrans) {
+    const size_t n2 = (size_t)g.n * g.n;
+    for (size_t i = 0; i < n2; ++i) {
+        T s = 0;
+        for (int z = 0; z < g.nz; ++z) s += V[(size_t)z * n2 + i];
+        const T ph = ((T)g.sigma * s) * (T)g.dx;
+        ptrans[i] = cx<T>(std::cos(ph), std::sin(ph));
+    }
+}
+
+// the operator pair of the selected method
+template <typename T> void trans_any(const Geo& g, const T* V, cx<T>* trans) {
+    if (g.sset) transmission_sset<T>(g, V, trans);
+    else transmission<T>(g, V, trans);
+}
+template <typename T>
+double forward_any(const Geo& g, const cx<T>* trans, const cx<T>* psi0, const cx<T>* prop, const T* M, cx<T>* save,
+                   cx<T>* resi) {
+    if (g.sset) return forward_sset<T>(g, trans, psi0, M, resi);
+    return forward<T>(g, trans, psi0, prop, M, save, nullptr, resi);
+}
+template <typename T>
+void back_any(const Geo& g, int mode, const cx<T>* trans, const cx<T>* save, const cx<T>* psi0, const cx<T>* resi,
+              const cx<T>* back_prop, cx<T>* grad, cx<T>* bw, double row, double col, double* dr, double* dc) {
+    if (g.sset) back_sset<T>(g, mode, trans, psi0, resi, grad, bw, dr, dc);
+    else back<T>(g, mode, trans, save, psi0, resi, back_prop, grad, bw, row, col, dr, dc);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -531,8 +605,11 @@ struct Work {
     }
 };
 
-Geo make_geo(int n, int nz, int bin, double sigma, double dx) {
+// mode bit 2 selects the SSET operator (one plane of depth nz); bit 1 the gradient mode
+Geo make_geo(int n, int nz, int bin, double sigma, double dx, int mode = 0) {
     Geo g;
+    g.sset = (mode & MODE_SSET) != 0;
+    if (g.sset) bin = nz;
     g.n = n; g.nz = nz; g.bin = std::max(1, std::min(bin, nz)); g.L = (nz + g.bin - 1) / g.bin;
     g.sigma = sigma; g.dx = dx;
     return g;
@@ -547,14 +624,14 @@ int run_tilt(const Geo& g, int mode, T* V, cx<T>* probe, const cx<T>* prop, cons
     std::vector<T> M(n2);
     for (int k = 0; k < npos; ++k) {
         const double row = xlist[k], col = ylist[k];
-        transmission<T>(g, V, w.trans.data());
+        trans_any<T>(g, V, w.trans.data());
         subpixel_shift<T>(probe, w.psi0.data(), g.n, row, col);
         for (size_t i = 0; i < n2; ++i) M[i] = (T)Mall[(size_t)k * n2 + i];
-        const double e = forward<T>(g, w.trans.data(), w.psi0.data(), prop, M.data(), w.save.data(), nullptr, w.resi.data());
+        const double e = forward_any<T>(g, w.trans.data(), w.psi0.data(), prop, M.data(), w.save.data(), w.resi.data());
         if (err_pos) err_pos[k] = e;
         if (err_acc) *err_acc += e / nsx / nsy;
         double dr, dc;
-        back<T>(g, mode, w.trans.data(), w.save.data(), w.psi0.data(), w.resi.data(), back_prop, w.grad.data(), w.bw.data(), row, col, &dr, &dc);
+        back_any<T>(g, mode, w.trans.data(), w.save.data(), w.psi0.data(), w.resi.data(), back_prop, w.grad.data(), w.bw.data(), row, col, &dr, &dc);
         probe_update<T>(probe, w.bw.data(), g.n, row, col, steps[1]);
         xlist[k] = (float)((T)row + (T)steps[2] * (T)dr);
         ylist[k] = (float)((T)col + (T)steps[2] * (T)dc);
@@ -573,7 +650,7 @@ int error_tilt(const Geo& g, const T* V, const cx<T>* probe, const cx<T>* prop, 
     // per-position terms are summed afterwards in position order
     const size_t n2 = (size_t)g.n * g.n;
     std::vector<cx<T>> trans(n2 * g.L);
-    transmission<T>(g, V, trans.data());
+    trans_any<T>(g, V, trans.data());
     std::vector<double> ep(npos);
 #pragma omp parallel
     {
@@ -583,7 +660,7 @@ int error_tilt(const Geo& g, const T* V, const cx<T>* probe, const cx<T>* prop, 
         for (int k = 0; k < npos; ++k) {
             subpixel_shift<T>(probe, w.psi0.data(), g.n, xlist[k], ylist[k]);
             for (size_t i = 0; i < n2; ++i) M[i] = (T)Mall[(size_t)k * n2 + i];
-            ep[k] = forward<T>(g, trans.data(), w.psi0.data(), prop, M.data(), w.save.data(), nullptr, nullptr);
+            ep[k] = forward_any<T>(g, trans.data(), w.psi0.data(), prop, M.data(), w.save.data(), nullptr);
         }
     }
     for (int k = 0; k < npos; ++k) {
@@ -604,7 +681,7 @@ int grad_fixed(const Geo& g, int mode, const T* V, const cx<T>* probe, const cx<
     Work<T> w(g);
     std::vector<T> M(n2);
     std::vector<cx<T>> sh(n2);
-    transmission<T>(g, V, w.trans.data());
+    trans_any<T>(g, V, w.trans.data());
     std::fill(dV, dV + n2 * g.L, T(0));
     std::fill(dprobe, dprobe + n2, cx<T>(0, 0));
     const double c2 = 2.0 * mround((g.n + 1) / 2.0);
@@ -612,10 +689,10 @@ int grad_fixed(const Geo& g, int mode, const T* V, const cx<T>* probe, const cx<
         const double row = xlist[k], col = ylist[k];
         subpixel_shift<T>(probe, w.psi0.data(), g.n, row, col);
         for (size_t i = 0; i < n2; ++i) M[i] = (T)Mall[(size_t)k * n2 + i];
-        const double e = forward<T>(g, w.trans.data(), w.psi0.data(), prop, M.data(), w.save.data(), nullptr, w.resi.data());
+        const double e = forward_any<T>(g, w.trans.data(), w.psi0.data(), prop, M.data(), w.save.data(), w.resi.data());
         if (err_pos) err_pos[k] = e;
         double dr, dc;
-        back<T>(g, mode, w.trans.data(), w.save.data(), w.psi0.data(), w.resi.data(), back_prop, w.grad.data(), w.bw.data(), row, col, &dr, &dc);
+        back_any<T>(g, mode, w.trans.data(), w.save.data(), w.psi0.data(), w.resi.data(), back_prop, w.grad.data(), w.bw.data(), row, col, &dr, &dc);
         for (int a = 0; a < g.L; ++a)
             for (size_t i = 0; i < n2; ++i) dV[(size_t)a * n2 + i] += std::real(w.grad[(size_t)bin_lo(g, a) * n2 + i]);
         subpixel_shift<T>(w.bw.data(), sh.data(), g.n, c2 - row, c2 - col);
@@ -633,12 +710,12 @@ double loss_fixed(const Geo& g, const T* V, const cx<T>* probe, const cx<T>* pro
     const size_t n2 = (size_t)g.n * g.n;
     Work<T> w(g);
     std::vector<T> M(n2);
-    transmission<T>(g, V, w.trans.data());
+    trans_any<T>(g, V, w.trans.data());
     double L = 0;
     for (int k = 0; k < npos; ++k) {
         subpixel_shift<T>(probe, w.psi0.data(), g.n, xlist[k], ylist[k]);
         for (size_t i = 0; i < n2; ++i) M[i] = (T)Mall[(size_t)k * n2 + i];
-        L += forward<T>(g, w.trans.data(), w.psi0.data(), prop, M.data(), w.save.data(), nullptr, nullptr) * (double)n2;
+        L += forward_any<T>(g, w.trans.data(), w.psi0.data(), prop, M.data(), w.save.data(), nullptr) * (double)n2;
     }
     return L;
 }
@@ -763,40 +840,40 @@ void oracle_back_d(int n, int nz, int bin, double sigma, double dx, int mode, co
 int oracle_run_tilt_d(int n, int nz, int bin, double sigma, double dx, int mode, double* V, double* probe,
                       const double* prop, const double* back_prop, float* xlist, float* ylist, const float* M,
                       int npos, int nsx, int nsy, const double* steps, double* err_acc, double* err_pos) {
-    return run_tilt<double>(make_geo(n, nz, bin, sigma, dx), mode, V, (cx<double>*)probe, (const cx<double>*)prop,
+    return run_tilt<double>(make_geo(n, nz, bin, sigma, dx, mode), mode & 1, V, (cx<double>*)probe, (const cx<double>*)prop,
                             (const cx<double>*)back_prop, xlist, ylist, M, npos, nsx, nsy, steps, err_acc, err_pos);
 }
 int oracle_run_tilt_f(int n, int nz, int bin, double sigma, double dx, int mode, float* V, float* probe,
                       const float* prop, const float* back_prop, float* xlist, float* ylist, const float* M,
                       int npos, int nsx, int nsy, const double* steps, double* err_acc, double* err_pos) {
-    return run_tilt<float>(make_geo(n, nz, bin, sigma, dx), mode, V, (cx<float>*)probe, (const cx<float>*)prop,
+    return run_tilt<float>(make_geo(n, nz, bin, sigma, dx, mode), mode & 1, V, (cx<float>*)probe, (const cx<float>*)prop,
                            (const cx<float>*)back_prop, xlist, ylist, M, npos, nsx, nsy, steps, err_acc, err_pos);
 }
 
-int oracle_error_tilt_d(int n, int nz, int bin, double sigma, double dx, const double* V, const double* probe,
+int oracle_error_tilt_d(int n, int nz, int bin, double sigma, double dx, int mode, const double* V, const double* probe,
                         const double* prop, const float* xlist, const float* ylist, const float* M, int npos,
                         int nsx, int nsy, double* err_acc, double* err_pos) {
-    return error_tilt<double>(make_geo(n, nz, bin, sigma, dx), V, (const cx<double>*)probe, (const cx<double>*)prop,
+    return error_tilt<double>(make_geo(n, nz, bin, sigma, dx, mode), V, (const cx<double>*)probe, (const cx<double>*)prop,
                               xlist, ylist, M, npos, nsx, nsy, err_acc, err_pos);
 }
-int oracle_error_tilt_f(int n, int nz, int bin, double sigma, double dx, const float* V, const float* probe,
+int oracle_error_tilt_f(int n, int nz, int bin, double sigma, double dx, int mode, const float* V, const float* probe,
                         const float* prop, const float* xlist, const float* ylist, const float* M, int npos,
                         int nsx, int nsy, double* err_acc, double* err_pos) {
-    return error_tilt<float>(make_geo(n, nz, bin, sigma, dx), V, (const cx<float>*)probe, (const cx<float>*)prop,
+    return error_tilt<float>(make_geo(n, nz, bin, sigma, dx, mode), V, (const cx<float>*)probe, (const cx<float>*)prop,
                              xlist, ylist, M, npos, nsx, nsy, err_acc, err_pos);
 }
 
 int oracle_grad_fixed_d(int n, int nz, int bin, double sigma, double dx, int mode, const double* V, const double* probe,
                         const double* prop, const double* back_prop, const double* xlist, const double* ylist,
                         const float* M, int npos, double* dV, double* dprobe, double* dpos, double* err_pos) {
-    return grad_fixed<double>(make_geo(n, nz, bin, sigma, dx), mode, V, (const cx<double>*)probe,
+    return grad_fixed<double>(make_geo(n, nz, bin, sigma, dx, mode), mode & 1, V, (const cx<double>*)probe,
                               (const cx<double>*)prop, (const cx<double>*)back_prop, xlist, ylist, M, npos, dV,
                               (cx<double>*)dprobe, dpos, err_pos);
 }
 
-double oracle_loss_d(int n, int nz, int bin, double sigma, double dx, const double* V, const double* probe,
+double oracle_loss_d(int n, int nz, int bin, double sigma, double dx, int mode, const double* V, const double* probe,
                      const double* prop, const double* xlist, const double* ylist, const float* M, int npos) {
-    return loss_fixed<double>(make_geo(n, nz, bin, sigma, dx), V, (const cx<double>*)probe, (const cx<double>*)prop,
+    return loss_fixed<double>(make_geo(n, nz, bin, sigma, dx, mode), V, (const cx<double>*)probe, (const cx<double>*)prop,
                               xlist, ylist, M, npos);
 }
 

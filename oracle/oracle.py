@@ -15,6 +15,7 @@ _LIB = None
 
 MODE_REF = 0     # STEP03_BACK.m:14-15 pairing (wave after plane a)
 MODE_EXACT = 1   # true adjoint (wave entering plane a), spectral position derivative
+MODE_SSET = 2    # flag: single-slice operator (STEP02_FORWARD_4D_SSET.m / STEP03_BACK_SSET.m)
 
 
 def build():
@@ -45,10 +46,10 @@ def lib():
             'oracle_back_d': ([I, I, I, D, D, I, P, P, P, P, P, P, P, D, D, P], None),
             'oracle_run_tilt_d': ([I, I, I, D, D, I, P, P, P, P, P, P, P, I, I, I, P, P, P], I),
             'oracle_run_tilt_f': ([I, I, I, D, D, I, P, P, P, P, P, P, P, I, I, I, P, P, P], I),
-            'oracle_error_tilt_d': ([I, I, I, D, D, P, P, P, P, P, P, I, I, I, P, P], I),
-            'oracle_error_tilt_f': ([I, I, I, D, D, P, P, P, P, P, P, I, I, I, P, P], I),
+            'oracle_error_tilt_d': ([I, I, I, D, D, I, P, P, P, P, P, P, I, I, I, P, P], I),
+            'oracle_error_tilt_f': ([I, I, I, D, D, I, P, P, P, P, P, P, I, I, I, P, P], I),
             'oracle_grad_fixed_d': ([I, I, I, D, D, I, P, P, P, P, P, P, P, I, P, P, P, P], I),
-            'oracle_loss_d': ([I, I, I, D, D, P, P, P, P, P, P, I], D),
+            'oracle_loss_d': ([I, I, I, D, D, I, P, P, P, P, P, P, I], D),
             'oracle_rot_matrix': ([P, P, P, D, D, D, I, P], I),
             'oracle_shear_plan': ([P, D, P, P, P], None),
             'oracle_rotate_d': ([P, P, I, I, I, P, D], None),
@@ -223,7 +224,7 @@ def run_tilt(V, probe_, prop, back_prop, xlist, ylist, M, nsx, nsy, binning, sig
     return V, pr, xl, yl, float(acc[0]), ep
 
 
-def error_tilt(V, probe_, prop, xlist, ylist, M, nsx, nsy, binning, sigma, dx, single=False):
+def error_tilt(V, probe_, prop, xlist, ylist, M, nsx, nsy, binning, sigma, dx, single=False, mode=MODE_REF):
     """Run_MSET_recon.m:207-222: forward + loss for every position of one tilt. Returns (err, err_pos)."""
     if single:
         V = _f32(V); pr = _c64(probe_); pp = _c64(prop); fn = lib().oracle_error_tilt_f
@@ -234,8 +235,8 @@ def error_tilt(V, probe_, prop, xlist, ylist, M, nsx, nsy, binning, sigma, dx, s
     npos = xl.shape[0]
     acc = np.zeros(1, np.float64)
     ep = np.zeros(npos, np.float64)
-    fn(n, nz, int(binning), sigma, dx, _p(V), _p(pr), _p(pp), _p(xl), _p(yl), _p(M), npos, int(nsx), int(nsy),
-       _p(acc), _p(ep))
+    fn(n, nz, int(binning), sigma, dx, int(mode), _p(V), _p(pr), _p(pp), _p(xl), _p(yl), _p(M), npos, int(nsx),
+       int(nsy), _p(acc), _p(ep))
     return float(acc[0]), ep
 
 
@@ -247,7 +248,7 @@ def grad_fixed(V, probe_, prop, back_prop, xlist, ylist, M, binning, sigma, dx, 
     """
     V = _f64(V)
     nz, n, _ = V.shape
-    L = -(-nz // binning)
+    L = 1 if mode & MODE_SSET else -(-nz // binning)
     xl = _f64(xlist); yl = _f64(ylist); M = _f32(M)
     npos = xl.shape[0]
     dV = np.empty((L, n, n), np.float64)
@@ -259,12 +260,12 @@ def grad_fixed(V, probe_, prop, back_prop, xlist, ylist, M, binning, sigma, dx, 
     return dV, dp, dpos, ep
 
 
-def loss(V, probe_, prop, xlist, ylist, M, binning, sigma, dx):
+def loss(V, probe_, prop, xlist, ylist, M, binning, sigma, dx, mode=MODE_REF):
     """L = sum_k sum_q (|Psi_k(q)| - sqrt(M_k(q)))^2 at a fixed state."""
     V = _f64(V)
     nz, n, _ = V.shape
     xl = _f64(xlist); yl = _f64(ylist)
-    return lib().oracle_loss_d(n, nz, int(binning), sigma, dx, _p(V), _p(_c128(probe_)), _p(_c128(prop)), _p(xl),
+    return lib().oracle_loss_d(n, nz, int(binning), sigma, dx, int(mode), _p(V), _p(_c128(probe_)), _p(_c128(prop)), _p(xl),
                                _p(yl), _p(_f32(M)), xl.shape[0])
 
 

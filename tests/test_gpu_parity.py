@@ -185,13 +185,61 @@ def test_mini_batch_of_one_equals_sequential(mset):
     assert abs(e2 / e1 - 1) < 1e-5
 
 
-def test_unsupported_method_and_bad_shapes_raise(mset):
+def test_bad_method_and_shapes_raise(mset):
     from mset_amd import lib
     c = small_case(n=32, nz=4, binning=1, npos=2, seed=1)
     _upload_case(mset, c)
     with pytest.raises(lib.MsetError) as e:
-        mset.run(method=lib.METHOD_SSET)
-    assert e.value.code == lib.MSET_EUNSUPPORTED
+        mset.run(method=7)
+    assert e.value.code == lib.MSET_EINVAL
     with pytest.raises(lib.MsetError):
         mset.upload(np.zeros((4, 32, 30), np.float32), c['probe'], c['prop'], c['xl'], c['yl'],
                     np.zeros((2, 32, 30), np.float32), c['sig'], c['dx'])
+    c40 = small_case(n=40, nz=2, binning=1, npos=1, seed=1)
+    with pytest.raises(lib.MsetError) as e:
+        _upload_case(mset, c40)
+    assert e.value.code == lib.MSET_EUNSUPPORTED
+
+
+# ------------------------------------------------------------------ SSET ('run_sset' / 'error_sset')
+@pytest.mark.parametrize('n,nz,binning,mode', [(32, 6, 2, O.MODE_REF), (64, 5, 1, O.MODE_EXACT),
+                                               (128, 8, 4, O.MODE_REF)])
+def test_sset_sequential_and_error_vs_oracle(mset, n, nz, binning, mode):
+    """STEP02_FORWARD_4D_SSET / STEP03_BACK_SSET with the sequential updates of Run_MSET_recon.m:134-158."""
+    from mset_amd import lib
+    c = small_case(n=n, nz=nz, binning=binning, npos=5, seed=7 * n + nz, spread=n / 8)
+    print('case: sset', n, nz, binning, mode, flush=True)
+    _upload_case(mset, c, mode=mode)
+    e = mset.error(method=lib.METHOD_SSET)
+    oe, _ = O.error_tilt(c['V'], c['probe'], c['prop'], c['xl'], c['yl'], c['M'], c['npos'], 1, c['bin'], c['sig'],
+                         c['dx'], mode=O.MODE_SSET)
+    assert abs(e / oe - 1) < LOSS_TOL
+    em = mset.error(method=lib.METHOD_MSET)
+    assert abs(em / e - 1) > 2 * LOSS_TOL             # the two operators differ on this state
+    steps = (30.0, 2e-3, 2e-2)
+    _upload_case(mset, c, steps=steps, mode=mode)
+    V, pr, xl, yl, err = mset.run(method=lib.METHOD_SSET)
+    oV, opr, oxl, oyl, oerr, _ = O.run_tilt(c['V'], c['probe'], c['prop'], c['bprop'], c['xl'], c['yl'], c['M'],
+                                            c['npos'], 1, c['bin'], c['sig'], c['dx'], steps, mode=mode | O.MODE_SSET)
+    assert abs(err / oerr - 1) < LOSS_TOL
+    assert_update_close(V, oV, c['V'], c['npos'])
+    assert_update_close(pr, opr, c['probe'], c['npos'])
+    dV = V - c['V']
+    assert np.abs(dV - dV[0]).max() <= 1e-5 * np.abs(dV).max() + 1e-5 * np.abs(c['V']).max()   # every slice
+    ulp = np.spacing(np.float32(max(np.abs(c['xl']).max(), np.abs(c['yl']).max())))
+    d, od = np.c_[xl - c['xl'], yl - c['yl']], np.c_[oxl - c['xl'], oyl - c['yl']]
+    assert np.abs(d - od).max() <= max(5e-3 * np.abs(od).max(), 4 * ulp)
+
+
+def test_sset_batched_step_is_the_fixed_state_gradient(mset):
+    """One mini-batch over every position: the object step is s1 * sum_k Re(grad_k) on every slice."""
+    from mset_amd import lib
+    c = small_case(n=64, nz=6, binning=3, npos=7, seed=21, spread=8)
+    steps = (30.0, 0.0, 0.0)
+    _upload_case(mset, c, steps=steps)
+    e = mset.run_batched(c['npos'], method=lib.METHOD_SSET)
+    V, _, _, _ = mset.download()
+    odV, _, _, oep = O.grad_fixed(c['V'], c['probe'], c['prop'], c['bprop'], c['xl'], c['yl'], c['M'], c['bin'],
+                                  c['sig'], c['dx'], mode=O.MODE_REF | O.MODE_SSET)
+    assert abs(e / (oep.sum() / c['npos']) - 1) < LOSS_TOL
+    assert_update_close(V, c['V'] + steps[0] * odV[0][None], c['V'], 1)

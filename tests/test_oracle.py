@@ -295,6 +295,57 @@ def test_reference_position_gradient_approximates_exact():
     assert rel(dr[:, ::-1], de) > 0.3 and rel(-dr, de) > 1.5       # discriminates axis swap and sign
 
 
+# ------------------------------------------------------------------ SSET (single-slice operator)
+def test_sset_is_one_plane_of_the_multislice_without_propagation():
+    """STEP02_FORWARD_4D_SSET / STEP03_BACK_SSET: the whole depth as one transmission and no propagator.
+    That is the multislice operator with slice_binning = nz and p = 1; the object term pairs the incident
+    wave (the exact-adjoint pairing) and the position terms use the same formula as STEP03_BACK."""
+    c = _small_case(nz=5, binning=2)
+    one = np.ones_like(c['prop'])
+    nz = c['V'].shape[0]
+    for mode in (O.MODE_REF, O.MODE_EXACT):
+        s = O.grad_fixed(c['V'], c['probe'], c['prop'], c['bprop'], c['xl'], c['yl'], c['M'], c['bin'], c['sig'],
+                         c['dx'], mode=mode | O.MODE_SSET)
+        m_obj = O.grad_fixed(c['V'], c['probe'], one, one, c['xl'], c['yl'], c['M'], nz, c['sig'], c['dx'],
+                             mode=O.MODE_EXACT)
+        m_pos = O.grad_fixed(c['V'], c['probe'], one, one, c['xl'], c['yl'], c['M'], nz, c['sig'], c['dx'], mode=mode)
+        assert s[0].shape == (1,) + c['V'].shape[1:]
+        assert rel(s[0], m_obj[0]) < 1e-10 and rel(s[1], m_obj[1]) < 1e-10
+        assert rel(s[2], m_pos[2]) < 1e-10 and rel(s[3], m_obj[3]) < 1e-10
+    # SSET pairs the incident wave even in reference mode: the multislice reference pairing (the wave
+    # after the plane) gives a different object term
+    m_ref = O.grad_fixed(c['V'], c['probe'], one, one, c['xl'], c['yl'], c['M'], nz, c['sig'], c['dx'],
+                         mode=O.MODE_REF)
+    assert rel(s[0], m_ref[0]) > 1e-2
+
+
+def test_sset_object_gradient_vs_central_fd():
+    c = _small_case(nz=4, binning=1)
+    n = c['n']
+    dV, _, _, _ = _grad(c, O.MODE_REF | O.MODE_SSET)
+    rng = np.random.default_rng(4)
+    h = 1e-2
+    lossf = lambda V: O.loss(V, c['probe'], c['prop'], c['xl'], c['yl'], c['M'], c['bin'], c['sig'], c['dx'],
+                             mode=O.MODE_SSET)
+    for _ in range(4):
+        d = rng.standard_normal(c['V'].shape)
+        fd = (lossf(c['V'] + h * d) - lossf(c['V'] - h * d)) / (2 * h)
+        an = -2 * n * n * c['sig'] * c['dx'] * np.sum(dV[0] * d)      # one plane, every slice
+        assert abs(fd - an) <= 1e-6 * abs(fd) + 1e-9, (fd, an)
+
+
+def test_sset_run_tilt_single_matches_double_and_updates_every_slice():
+    c = _small_case(nz=4, binning=2, npos=3)
+    steps = (30.0, 2e-3, 2e-2)
+    out_d = O.run_tilt(c['V'], c['probe'], c['prop'], c['bprop'], c['xl'], c['yl'], c['M'], 3, 1, c['bin'], c['sig'],
+                       c['dx'], steps, mode=O.MODE_SSET)
+    out_f = O.run_tilt(c['V'], c['probe'], c['prop'], c['bprop'], c['xl'], c['yl'], c['M'], 3, 1, c['bin'], c['sig'],
+                       c['dx'], steps, mode=O.MODE_SSET, single=True)
+    dV = out_d[0] - c['V']
+    assert np.abs(dV - dV[0]).max() <= 1e-12 * np.abs(dV).max()       # grad is replicated over all slices
+    assert rel(out_f[0] - c['V'], dV) < 1e-4 and abs(out_f[4] / out_d[4] - 1) < 1e-5
+
+
 def test_adjoint_identity_of_one_plane():
     """<A u, y> = <u, A^H y> for A u = ifft2(p . fft2(t . u)) (the forward plane operator)."""
     c = _small_case(nz=1, binning=1)
