@@ -1,72 +1,86 @@
 // fft_reg.hpp -- register-resident 2D FFT of a 128 x 128 complex plane for CDNA4 (gfx950).
 //
-// The workgroup has 512 threads.  Four consecutive lanes (a quad) own one line of the plane (a
-// column in the "C" orientation, a row in the "R" orientation), 32 elements each, in registers.  A
-// 1D transform of a 128-point line is a 32-point DFT in registers, a twiddle multiply, and a 4-point
-// DFT across the quad done with DPP lane swaps (quad_perm xor 1 / xor 2, no LDS, no barrier).  The
-// twiddles W_128^(b(h) k) come from the LDS table (the four lanes of a quad read four entries, all
-// quads the same four: a broadcast):
+// The workgroup has 512 threads (8 waves).  In a wave, lane = 16 h + c: the four lanes of one column c
+// (one per 16-lane row h) own one line of the plane (a column in the "C" orientation, a row in the "R"
+// orientation), 32 elements each, in registers; a wave owns 16 lines, line = 16 wave + c.
 //
-//   decimation in time (forward):  lane h holds x[b(h) + 4m]   ->  lane h holds X[32 h + k]
-//   decimation in frequency (inv): lane h holds x[32 h + m]    ->  lane h holds X[b(h) + 4 k]
+// A 128-point transform is split 128 = 4 x 32.  Decimation in time (forward):
+//   row h holds x[h + 4 m] (register m)                                   layout D
+//   -> 32-point DFT over m in registers, twiddle W_128^(h k)
+//   -> per group of four registers k = 4 g .. 4 g + 3, a 4 x 4 transpose across the rows with two
+//      v_permlane32_swap and two v_permlane16_swap per component (row p ends up with the four rows'
+//      values of k = 4 g + p) and a 4-point DFT in registers:
+//   row h holds X[32 j + 4 g + h] in register 4 g + j                     layout X
+// Decimation in frequency (inverse) runs the same steps backwards and maps layout X to layout D, so a
+// forward transform's output is exactly what the inverse consumes and vice versa: consecutive
+// transforms never reshuffle registers.  No LDS, no barrier and no lane-dependent select is involved
+// in a 1D transform; the twiddles W_128^(h k) come from an LDS table (the 16 lanes of a row read the
+// same entry: a broadcast).
 //
-// with b the 2-bit bit reversal.  The two layouts chain: a forward transform ends with blocked lines
-// and the inverse starts from them, and the inverse ends with decimated lines, which is what the next
-// forward transform consumes, so consecutive transforms never reshuffle registers.  The only LDS
-// traffic of a 2D transform is one transpose between its column and row passes (C <-> R), through a
-// 135 KB buffer whose swizzle A(y, x) = 132 y + x + 4 (y >> 5) keeps every 16-lane access on 16
-// distinct 8-byte bank groups for all four access patterns.
+// A 2D transform is a column pass, one transpose through LDS (C <-> R) and a row pass.  The transpose
+// buffer has an odd row pitch (129 float2): the 16 lanes of a row access either 16 consecutive
+// elements of one row or the same element of 16 consecutive rows, both free of bank conflicts.
 //
 // Interface (shared with LdsEngine in fft_lds.hpp): fwd(pre, post) goes C -> R and inv(pre, post)
 // goes R -> C; pre / post see (y, x, value) for every element the thread owns; each_C / each_R visit
-// the owned elements after an inverse / forward transform.  In C, consecutive lanes hold consecutive
-// x (coalesced for row-major [y][x] arrays); in R, consecutive lanes hold consecutive y (coalesced for
-// arrays stored [x][y]).
+// the owned elements after an inverse / forward transform.  In C, the 16 lanes of a row hold 16
+// consecutive x (coalesced for row-major [y][x] arrays); in R they hold 16 consecutive y (coalesced
+// for arrays stored [x][y]).
 #pragma once
 #include <hip/hip_runtime.h>
 #include "fft_lds.hpp"
 
 namespace mset {
 
-template <int CTRL> __device__ __forceinline__ float dpp_f(float v) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));
+// exchange across 16-lane rows: rows 2, 3 of a <-> rows 0, 1 of b (v_permlane32_swap), or odd rows of
+// a <-> even rows of b (v_permlane16_swap)
+__device__ __forceinline__ void swap32(float& a, float& b) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(unsigned, a), __builtin_bit_cast(unsigned, b),
+                                                    false, false);
+    a = __builtin_bit_cast(float, (unsigned)r[0]);
+    b = __builtin_bit_cast(float, (unsigned)r[1]);
 }
-template <int CTRL> __device__ __forceinline__ float2 dpp_c(float2 v) {
-    return make_float2(dpp_f<CTRL>(v.x), dpp_f<CTRL>(v.y));
+__device__ __forceinline__ void swap16(float& a, float& b) {
+    const auto r = __builtin_amdgcn_permlane16_swap(__builtin_bit_cast(unsigned, a), __builtin_bit_cast(unsigned, b),
+                                                    false, false);
+    a = __builtin_bit_cast(float, (unsigned)r[0]);
+    b = __builtin_bit_cast(float, (unsigned)r[1]);
 }
-constexpr int DPP_XOR1 = 0xB1;   // quad_perm [1, 0, 3, 2]
-constexpr int DPP_XOR2 = 0x4E;   // quad_perm [2, 3, 0, 1]
 
-// lane-sign combine: lanes with the bit clear keep own + partner, the others partner - own
-__device__ __forceinline__ float2 bfly(float2 own, float2 partner, float sgn) {
-    return make_float2(fmaf(sgn, own.x, partner.x), fmaf(sgn, own.y, partner.y));
+// 4 x 4 transpose across the rows of a wave: before, row p register q holds M[p][q]; after, row p
+// register q holds M[q][p]
+__device__ __forceinline__ void transpose4(float2* v) {
+    swap32(v[0].x, v[2].x);
+    swap32(v[0].y, v[2].y);
+    swap32(v[1].x, v[3].x);
+    swap32(v[1].y, v[3].y);
+    swap16(v[0].x, v[1].x);
+    swap16(v[0].y, v[1].y);
+    swap16(v[2].x, v[3].x);
+    swap16(v[2].y, v[3].y);
 }
 
 template <int N> struct RegEngine;
 
 template <> struct RegEngine<128> {
-    static constexpr int N = 128, E = 32, NT = 512, SP = 132;
-    static constexpr int LDS_ELEMS = 127 * SP + 127 + 4 * 3 + 1;
+    static constexpr int N = 128, E = 32, NT = 512, SP = 129;
+    static constexpr int LDS_ELEMS = N * SP;
     float2* s;
-    const float2* tw;  // W_128^k table in LDS; lanes of one b(h) read the same entry (broadcast)
+    const float2* tw;  // W_128^k table in LDS
     float2 v[E];
-    int line, h, b;
-    float s1, s2;      // lane signs for the xor-1 and xor-2 butterflies
-    bool lane3;
+    int line, h;
 
     __device__ __forceinline__ void init(float2* lds, const float2* lds_tw) {
         s = lds;
         tw = lds_tw;
         const int t = threadIdx.x;
-        line = t >> 2;
-        h = t & 3;
-        b = ((h & 1) << 1) | (h >> 1);
-        s1 = (h & 1) ? -1.f : 1.f;
-        s2 = (h & 2) ? -1.f : 1.f;
-        lane3 = h == 3;
+        line = ((t >> 6) << 4) | (t & 15);
+        h = (t >> 4) & 3;
     }
 
-    static __device__ __forceinline__ int A(int y, int x) { return y * SP + x + 4 * (y >> 5); }
+    static __device__ __forceinline__ int A(int y, int x) { return y * SP + x; }
+    // element index along the line of register i in layout X (layout D: h + 4 i)
+    __device__ __forceinline__ int xidx(int i) const { return 32 * (i & 3) + 4 * (i >> 2) + h; }
 
     // The element functors load global operands; unrolled over 32 elements the scheduler would hoist
     // every load ahead of its use and run out of registers.  A scheduling fence after every G
@@ -75,82 +89,72 @@ template <> struct RegEngine<128> {
         if ((i & (G - 1)) == G - 1) __builtin_amdgcn_sched_barrier(0);
     }
 
-    // Per-element addresses are cheap affine functions of (line, h, b).  Left visible, the compiler
-    // hoists all of them out of the plane and position loops and keeps hundreds of registers live;
-    // routing the coordinates through an empty asm at every transform makes them opaque, so the
-    // addresses are recomputed where they are used.
-    __device__ __forceinline__ void opaque() {
-        asm volatile("" : "+v"(line), "+v"(h), "+v"(b));
-    }
+    // Per-element addresses are cheap affine functions of (line, h).  Left visible, the compiler hoists
+    // all of them out of the plane and position loops and keeps hundreds of registers live; routing the
+    // coordinates through an empty asm at every transform makes them opaque, so the addresses are
+    // recomputed where they are used.
+    __device__ __forceinline__ void opaque() { asm volatile("" : "+v"(line), "+v"(h)); }
 
-    // lanes hold z_{b(h)}; afterwards lane h holds sum_j z_j W4^(j h)
-    template <bool INV> __device__ __forceinline__ float2 xdit(float2 z) const {
-        z = bfly(z, dpp_c<DPP_XOR1>(z), s1);
-        z = lane3 ? mul_mi<INV>(z) : z;
-        return bfly(z, dpp_c<DPP_XOR2>(z), s2);
-    }
-    // lanes hold z_h; afterwards lane h holds sum_j z_j W4^(j b(h))
-    template <bool INV> __device__ __forceinline__ float2 xdif(float2 z) const {
-        z = bfly(z, dpp_c<DPP_XOR2>(z), s2);
-        z = lane3 ? mul_mi<INV>(z) : z;
-        return bfly(z, dpp_c<DPP_XOR1>(z), s1);
-    }
-
+    // layout D -> layout X
     template <bool INV> __device__ __forceinline__ void pass_dit() {
         DFT<E, INV>::run(v);
 #pragma unroll
         for (int k = 1; k < E; ++k) {
-            const float2 w = tw[b * k];
+            const float2 w = tw[h * k];
             v[k] = INV ? cmulc(v[k], w) : cmul(v[k], w);
             group_fence<8>(k);
         }
 #pragma unroll
-        for (int k = 0; k < E; ++k) {
-            v[k] = xdit<INV>(v[k]);
-            group_fence<8>(k);
+        for (int g = 0; g < E / 4; ++g) {
+            transpose4(v + 4 * g);
+            DFT<4, INV>::run(v + 4 * g);
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
+    // layout X -> layout D
     template <bool INV> __device__ __forceinline__ void pass_dif() {
 #pragma unroll
-        for (int m = 0; m < E; ++m) {
-            v[m] = xdif<INV>(v[m]);
-            group_fence<8>(m);
+        for (int g = 0; g < E / 4; ++g) {
+            DFT<4, INV>::run(v + 4 * g);
+            transpose4(v + 4 * g);
+            __builtin_amdgcn_sched_barrier(0);
         }
 #pragma unroll
         for (int m = 1; m < E; ++m) {
-            const float2 w = tw[b * m];
+            const float2 w = tw[h * m];
             v[m] = INV ? cmulc(v[m], w) : cmul(v[m], w);
             group_fence<8>(m);
         }
         DFT<E, INV>::run(v);
     }
 
-    // column-blocked (y = 32 h + k of column `line`) -> row-decimated (x = b + 4 m of row `line`)
+    // C orientation, layout X (y = xidx, x = line) -> R orientation, layout D (y = line, x = h + 4 m)
     __device__ __forceinline__ void transpose_c_to_r() {
         __syncthreads();
 #pragma unroll
-        for (int k = 0; k < E; ++k) s[A(32 * h + k, line)] = v[k];
+        for (int i = 0; i < E; ++i) s[A(xidx(i), line)] = v[i];
         __syncthreads();
 #pragma unroll
-        for (int m = 0; m < E; ++m) v[m] = s[A(line, b + 4 * m)];
+        for (int m = 0; m < E; ++m) v[m] = s[A(line, h + 4 * m)];
     }
-    // row-decimated (x = b + 4 k of row `line`) -> column-blocked (y = 32 h + m of column `line`)
+    // R orientation, layout D (y = line, x = h + 4 m) -> C orientation, layout X (y = xidx, x = line)
     __device__ __forceinline__ void transpose_r_to_c() {
         __syncthreads();
 #pragma unroll
-        for (int k = 0; k < E; ++k) s[A(line, b + 4 * k)] = v[k];
+        for (int m = 0; m < E; ++m) s[A(line, h + 4 * m)] = v[m];
         __syncthreads();
 #pragma unroll
-        for (int m = 0; m < E; ++m) v[m] = s[A(32 * h + m, line)];
+        for (int i = 0; i < E; ++i) v[i] = s[A(xidx(i), line)];
     }
 
-    // Visit the 32 owned elements.  Column-decimated coordinates (COL): element m is (b + 4 m, line);
-    // row-blocked ones: element k is (line, 32 h + k).  MAP: the functor returns the new value (LOADG:
-    // it is handed zero instead of the current one); otherwise it sees the element by reference.
+    // Visit the 32 owned elements.  COL: C orientation, layout D, element i is (h + 4 i, line);
+    // otherwise R orientation, layout X, element i is (line, xidx(i)).  MAP: the functor returns the
+    // new value (LOADG: it is handed zero instead of the current one); otherwise it sees the element
+    // by reference.
     template <bool COL, bool MAP, bool LOADG, class F> __device__ __forceinline__ void run_elems(const F& f) {
         auto yx = [&](int i, int& y, int& x) {
-            if (COL) { y = b + 4 * i; x = line; }
-            else { y = line; x = 32 * h + i; }
+            if (COL) { y = h + 4 * i; x = line; }
+            else { y = line; x = xidx(i); }
         };
         if constexpr (IsStaged<F>::value) {
             constexpr int G = F::G;
