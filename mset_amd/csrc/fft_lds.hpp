@@ -13,9 +13,9 @@
 // float2, and a row-pass access strides by the odd pitch S, which spreads 16 consecutive lanes over all
 // banks: neither pass has bank conflicts, whatever the stage stride.
 //
-// Element-wise work is fused into the transform instead of costing LDS sweeps of its own: `pre(y, x, v)`
-// is applied to every element as the first pass loads it (with LOADG the functor supplies the value
-// from global memory and LDS is not read) and `post(y, x, v)` as the last pass stores it.  A forward
+// Element-wise work is fused into the transform instead of costing LDS sweeps of its own: `pre(p, v)`
+// is applied to every element p as the first pass loads it (with LOADG the functor supplies the value
+// from global memory and LDS is not read) and `post(p, v)` as the last pass stores it.  A forward
 // transform (FWD_CR) runs columns then rows, an inverse (INV_RC) rows then columns.  In a column pass
 // consecutive lanes see consecutive x, so functors touching row-major [y][x] global arrays are
 // coalesced; in a row pass consecutive lanes see consecutive y, so functors there should touch arrays
@@ -142,26 +142,66 @@ template <bool INV> struct DFT<3, INV> {
     }
 };
 
-// ------------------------------------------------------------------ functors
-struct NoOp {
-    __device__ __forceinline__ float2 operator()(int, int, float2 v) const { return v; }
+// ------------------------------------------------------------------ element points and buffer views
+// An element as the engines hand it to functors: its coordinates and its index in the layout the
+// current orientation streams (row-major [y][x] when consecutive lanes run along x, transposed [x][y]
+// when they run along y).  The index is split as v + s, v varying per lane and s a constant of the
+// unrolled element loop, so that buffer accesses take s as a scalar offset and cost no VALU.
+struct Idx {
+    unsigned v, s;
+};
+struct Pt {
+    int y, x;
+    Idx e;
 };
 
-// An element functor split in two phases: load(y, x) fetches the element's global operands, apply(y, x,
-// v, op) combines them with the element.  An engine that holds many elements per thread issues the loads
+// Raw buffer view of a global array: one scalar descriptor; an access at Idx (v, s) is a buffer
+// instruction with VGPR offset v and SGPR offset s.  The hardware bounds-checks every access against
+// the view's size: reads beyond it return 0 and writes beyond it are dropped.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+typedef unsigned u32x2 __attribute__((__vector_size__(8)));   // the type the 64-bit buffer builtins use
+// (the pair is converted whole with bit_cast: element access on this vector type reads element 0 twice)
+struct BufC {   // complex (float2) elements
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ BufC(const float2* p, unsigned n) : r(buffer_rsrc(p, n * 8u)) {}
+    __device__ __forceinline__ float2 operator[](Idx e) const {
+        return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, e.v * 8u, e.s * 8u, 0));
+    }
+    __device__ __forceinline__ void put(Idx e, float2 a) const {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, a), r, e.v * 8u, e.s * 8u, 0);
+    }
+};
+struct BufF {   // real (float) elements
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ BufF(const float* p, unsigned n) : r(buffer_rsrc(p, n * 4u)) {}
+    __device__ __forceinline__ float operator[](Idx e) const {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, e.v * 4u, e.s * 4u, 0));
+    }
+    __device__ __forceinline__ void put(Idx e, float a) const {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, a), r, e.v * 4u, e.s * 4u, 0);
+    }
+};
+
+// ------------------------------------------------------------------ functors
+struct NoOp {
+    __device__ __forceinline__ float2 operator()(const Pt&, float2 v) const { return v; }
+};
+
+// An element functor split in two phases: load(p) fetches the element's global operands, apply(p, v,
+// op) combines them with the element.  An engine that holds many elements per thread issues the loads
 // of G elements back to back before the first apply, so their memory latencies overlap instead of
 // adding up.  Called as an ordinary functor, it runs both phases for one element.
 template <int G_, class Ld, class Ap> struct Staged {
     static constexpr int G = G_;
     Ld ld;
     Ap ap;
-    __device__ __forceinline__ auto load(int y, int x) const { return ld(y, x); }
-    template <class V, class Op> __device__ __forceinline__ auto apply(int y, int x, V&& v, const Op& o) const {
-        return ap(y, x, v, o);
+    __device__ __forceinline__ auto load(const Pt& p) const { return ld(p); }
+    template <class V, class Op> __device__ __forceinline__ auto apply(const Pt& p, V&& v, const Op& o) const {
+        return ap(p, v, o);
     }
-    template <class V> __device__ __forceinline__ auto operator()(int y, int x, V&& v) const {
-        return ap(y, x, v, ld(y, x));
-    }
+    template <class V> __device__ __forceinline__ auto operator()(const Pt& p, V&& v) const { return ap(p, v, ld(p)); }
 };
 template <int G, class Ld, class Ap> __device__ __forceinline__ Staged<G, Ld, Ap> staged(Ld ld, Ap ap) {
     return Staged<G, Ld, Ap>{ld, ap};
@@ -193,8 +233,9 @@ __device__ __forceinline__ void fft_stage(float2* __restrict__ s, const float2* 
                 const int e = j + r * NL;
                 const int y = ROWS ? line : e, x = ROWS ? e : line;
                 float2 val;
-                if constexpr (FIRST && LOADG) val = pre(y, x, make_float2(0.f, 0.f));
-                else if constexpr (FIRST) val = pre(y, x, s[y * S + x]);
+                const Pt pt{y, x, Idx{(unsigned)(ROWS ? x * N + y : y * N + x), 0u}};
+                if constexpr (FIRST && LOADG) val = pre(pt, make_float2(0.f, 0.f));
+                else if constexpr (FIRST) val = pre(pt, s[y * S + x]);
                 else val = s[y * S + x];
                 v[c][r] = val;
             }
@@ -222,7 +263,7 @@ __device__ __forceinline__ void fft_stage(float2* __restrict__ s, const float2* 
                 const int e = base + r * NS;
                 const int y = ROWS ? line : e, x = ROWS ? e : line;
                 float2 val = v[c][r];
-                if constexpr (LAST) val = post(y, x, val);
+                if constexpr (LAST) val = post(Pt{y, x, Idx{(unsigned)(ROWS ? x * N + y : y * N + x), 0u}}, val);
                 s[y * S + x] = val;
             }
         }
@@ -280,14 +321,14 @@ template <int N, int NT_> struct LdsEngine {
     template <class F> __device__ __forceinline__ void each_R(const F& f) {
         for (int i = threadIdx.x; i < N * N; i += NT) {
             const int y = i % N, x = i / N;
-            f(y, x, s[y * S + x]);
+            f(Pt{y, x, Idx{(unsigned)i, 0u}}, s[y * S + x]);
         }
         __syncthreads();
     }
     template <class F> __device__ __forceinline__ void each_C(const F& f) {
         for (int i = threadIdx.x; i < N * N; i += NT) {
             const int y = i / N, x = i % N;
-            f(y, x, s[y * S + x]);
+            f(Pt{y, x, Idx{(unsigned)i, 0u}}, s[y * S + x]);
         }
         __syncthreads();
     }

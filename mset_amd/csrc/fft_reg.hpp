@@ -22,7 +22,7 @@
 // elements of one row or the same element of 16 consecutive rows, both free of bank conflicts.
 //
 // Interface (shared with LdsEngine in fft_lds.hpp): fwd(pre, post) goes C -> R and inv(pre, post)
-// goes R -> C; pre / post see (y, x, value) for every element the thread owns; each_C / each_R visit
+// goes R -> C; pre / post see (point, value) for every element the thread owns; each_C / each_R visit
 // the owned elements after an inverse / forward transform.  In C, the 16 lanes of a row hold 16
 // consecutive x (coalesced for row-major [y][x] arrays); in R they hold 16 consecutive y (coalesced
 // for arrays stored [x][y]).
@@ -147,44 +147,38 @@ template <> struct RegEngine<128> {
         for (int i = 0; i < E; ++i) v[i] = s[A(xidx(i), line)];
     }
 
-    // Visit the 32 owned elements.  COL: C orientation, layout D, element i is (h + 4 i, line);
-    // otherwise R orientation, layout X, element i is (line, xidx(i)).  MAP: the functor returns the
-    // new value (LOADG: it is handed zero instead of the current one); otherwise it sees the element
-    // by reference.
+    // Visit the 32 owned elements.  COL: C orientation, layout D, element i is (h + 4 i, line), index
+    // (h N + line) + 4 N i in a row-major [y][x] array; otherwise R orientation, layout X, element i is
+    // (line, xidx(i)), index (h N + line) + (32 (i & 3) + 4 (i >> 2)) N in a transposed [x][y] array.
+    // MAP: the functor returns the new value (LOADG: it is handed zero instead of the current one);
+    // otherwise it sees the element by reference.
     template <bool COL, bool MAP, bool LOADG, class F> __device__ __forceinline__ void run_elems(const F& f) {
-        auto yx = [&](int i, int& y, int& x) {
-            if (COL) { y = h + 4 * i; x = line; }
-            else { y = line; x = xidx(i); }
+        const unsigned vb = (unsigned)(h * N + line);
+        auto pt = [&](int i) -> Pt {
+            if (COL) return Pt{h + 4 * i, line, Idx{vb, (unsigned)(4 * N * i)}};
+            return Pt{line, xidx(i), Idx{vb, (unsigned)((32 * (i & 3) + 4 * (i >> 2)) * N)}};
         };
         if constexpr (IsStaged<F>::value) {
             constexpr int G = F::G;
             static_assert(E % G == 0, "group size must divide 32");
 #pragma unroll
             for (int g0 = 0; g0 < E; g0 += G) {
-                decltype(f.load(0, 0)) op[G];
+                decltype(f.load(pt(0))) op[G];
 #pragma unroll
-                for (int i = 0; i < G; ++i) {
-                    int y, x;
-                    yx(g0 + i, y, x);
-                    op[i] = f.load(y, x);
-                }
+                for (int i = 0; i < G; ++i) op[i] = f.load(pt(g0 + i));
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int i = 0; i < G; ++i) {
-                    int y, x;
-                    yx(g0 + i, y, x);
-                    if constexpr (MAP) v[g0 + i] = f.apply(y, x, LOADG ? make_float2(0.f, 0.f) : v[g0 + i], op[i]);
-                    else f.apply(y, x, v[g0 + i], op[i]);
+                    if constexpr (MAP) v[g0 + i] = f.apply(pt(g0 + i), LOADG ? make_float2(0.f, 0.f) : v[g0 + i], op[i]);
+                    else f.apply(pt(g0 + i), v[g0 + i], op[i]);
                 }
                 __builtin_amdgcn_sched_barrier(0);
             }
         } else {
 #pragma unroll
             for (int i = 0; i < E; ++i) {
-                int y, x;
-                yx(i, y, x);
-                if constexpr (MAP) v[i] = f(y, x, LOADG ? make_float2(0.f, 0.f) : v[i]);
-                else f(y, x, v[i]);
+                if constexpr (MAP) v[i] = f(pt(i), LOADG ? make_float2(0.f, 0.f) : v[i]);
+                else f(pt(i), v[i]);
                 group_fence(i);
             }
         }

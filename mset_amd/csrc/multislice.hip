@@ -155,56 +155,52 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
         __syncthreads();
     }
 
+    // buffer views (hardware bounds-checked); p.e indexes the layout the engine's orientation streams
+    const BufC pTb(pT, N2);                                 // propagator, [x][y]
+    const BufC sv0(save, 2 * N2);                           // psi0 and (single slice) the exit wave, [y][x]
+    const BufC F0b(F0, N2);                                 // Fourier psi0, [x][y]
+    const BufC tr0(A.trans, N2);                            // transmission of plane 0, [y][x]
+
     // ---- incident wave psi0 = ifftn(fftn(probe) .* Phi)
+    auto store_psi0 = [&](const Pt& p, float2 v) {
+        float2 w = cscale(v, inv_n2);
+        if (KIND != K_FWD) sv0.put(p.e, w);
+        if (A.sset) {                                       // single slice: exit wave psi0 .* t, no propagation
+            w = cmul(w, tr0[p.e]);
+            if (KIND != K_FWD) sv0.put(Idx{p.e.v, p.e.s + (unsigned)N2}, w);
+        }
+        return w;
+    };
     if constexpr (KIND == K_PLAIN) {
-        const float2* probe = A.probe;
-        E.template fwd<true>(staged<16>([&](int y, int x) { return probe[y * N + x]; },
-                                        [&](int, int, float2, float2 o) { return o; }),
-                             [&](int y, int x, float2 v) {
-                                 const float2 f = cmul(v, phase(y, x));
-                                 F0[x * N + y] = f;
+        const BufC pr(A.probe, N2);
+        E.template fwd<true>(staged<16>([&](const Pt& p) { return pr[p.e]; },
+                                        [&](const Pt&, float2, float2 o) { return o; }),
+                             [&](const Pt& p, float2 v) {
+                                 const float2 f = cmul(v, phase(p.y, p.x));
+                                 F0b.put(p.e, f);
                                  return f;
                              });
-        E.inv(NoOp(), [&](int y, int x, float2 v) {
-            float2 w = cscale(v, inv_n2);
-            save[y * N + x] = w;
-            if (A.sset) {                                   // single slice: exit wave psi0 .* t, no propagation
-                w = cmul(w, A.trans[y * N + x]);
-                save[N2 + y * N + x] = w;
-            }
-            return w;
-        });
+        E.inv(NoOp(), store_psi0);
     } else {
-        const float2* pfT = A.probeF;                       // fftn(probe), [x][y]
-        E.template inv<true>(
-                               staged<16>([&](int y, int x) { return pfT[x * N + y]; },
-                                          [&](int y, int x, float2, float2 o) {
-                                              const float2 f = cmul(o, phase(y, x));
-                                              if (KIND == K_BATCH) F0[x * N + y] = f;
-                                              return f;
-                                          }),
-                               [&](int y, int x, float2 v) {
-                                   float2 w = cscale(v, inv_n2);
-                                   if (KIND == K_BATCH) save[y * N + x] = w;
-                                   if (A.sset) {            // single slice: exit wave psi0 .* t, no propagation
-                                       w = cmul(w, A.trans[y * N + x]);
-                                       if (KIND == K_BATCH) save[N2 + y * N + x] = w;
-                                   }
-                                   return w;
-                               });
+        const BufC pf(A.probeF, N2);                        // fftn(probe), [x][y]
+        E.template inv<true>(staged<16>([&](const Pt& p) { return pf[p.e]; },
+                                        [&](const Pt& p, float2, float2 o) {
+                                            const float2 f = cmul(o, phase(p.y, p.x));
+                                            if (KIND == K_BATCH) F0b.put(p.e, f);
+                                            return f;
+                                        }),
+                             store_psi0);
     }
 
     // ---- forward sweep: psi_a = ifftn(fftn(psi_{a-1} .* t_a) .* p)
     for (int a = 0; a < (A.sset ? 0 : L); ++a) {
-        const float2* ta = A.trans + (size_t)a * N2;
-        E.fwd(staged<16>([&](int y, int x) { return ta[y * N + x]; },
-                         [&](int, int, float2 v, float2 o) { return cmul(v, o); }),
-              staged<16>([&](int y, int x) { return pT[x * N + y]; },
-                         [&](int, int, float2 v, float2 o) { return cmul(v, o); }));
-        float2* sv = save + (size_t)(a + 1) * N2;
-        E.inv(NoOp(), [&](int y, int x, float2 v) {
+        const BufC ta(A.trans + (size_t)a * N2, N2);
+        E.fwd(staged<16>([&](const Pt& p) { return ta[p.e]; }, [&](const Pt&, float2 v, float2 o) { return cmul(v, o); }),
+              staged<16>([&](const Pt& p) { return pTb[p.e]; }, [&](const Pt&, float2 v, float2 o) { return cmul(v, o); }));
+        const BufC sv(save + (size_t)(a + 1) * N2, N2);
+        E.inv(NoOp(), [&](const Pt& p, float2 v) {
             const float2 w = cscale(v, inv_n2);
-            if (KIND != K_FWD) sv[y * N + x] = w;
+            if (KIND != K_FWD) sv.put(p.e, w);
             return w;
         });
     }
@@ -212,18 +208,19 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
     // ---- exit wave, intensity, amplitude residual (STEP02_FORWARD_4D.m:36-49), r .* conj(p) fused
     double err = 0.0;
     {
-        const float* MT = A.sqrtM + (size_t)k * N2;         // sqrt(M), [x][y]
+        const BufF MT(A.sqrtM + (size_t)k * N2, N2);        // sqrt(M), [x][y]
         float* Iout = A.inten ? A.inten + (size_t)k * N2 : nullptr;
         E.fwd(NoOp(), staged<16>(
-                          [&](int y, int x) {
-                              const float m = MT[x * N + y];
+                          [&](const Pt& p) {
+                              const float m = MT[p.e];
                               if (KIND == K_FWD) return make_float3(m, 0.f, 0.f);
                               if (A.sset) return make_float3(m, 1.f, 0.f);     // no back-propagator
-                              return make_float3(m, pT[x * N + y].x, pT[x * N + y].y);
+                              const float2 q = pTb[p.e];
+                              return make_float3(m, q.x, q.y);
                           },
-                          [&](int y, int x, float2 v, float3 o) {
+                          [&](const Pt& p, float2 v, float3 o) {
                               const float I = v.x * v.x + v.y * v.y;
-                              if (KIND == K_FWD && Iout) Iout[((y + N / 2) % N) * N + (x + N / 2) % N] = I;
+                              if (KIND == K_FWD && Iout) Iout[((p.y + N / 2) % N) * N + (p.x + N / 2) % N] = I;
                               const float m = o.x;
                               // hardware square root and reciprocal square root (1 ulp)
                               const float d = m - __builtin_amdgcn_sqrtf(I);
@@ -240,9 +237,9 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
     // ---- adjoint sweep (STEP03_BACK.m:12-24, mset.cu:232-264)
     for (int a = L - 1; a >= 0; --a) {
         // the single-slice operator pairs the incident wave (STEP03_BACK_SSET.m:13-14)
-        const float2* ws = save + (size_t)((exact || A.sset) ? a : a + 1) * N2;
-        const float2* ta = A.trans + (size_t)a * N2;
-        float* g = A.gslab + ((size_t)slot * L + a) * N2;
+        const BufC ws(save + (size_t)((exact || A.sset) ? a : a + 1) * N2, N2);
+        const BufC ta(A.trans + (size_t)a * N2, N2);
+        const BufF g(A.gslab + ((size_t)slot * L + a) * N2, N2);
         auto adjoint = [&](float2 v, float2 t, float2 w, float& gr) {
             const float2 b = cscale(v, inv_n2);
             const float2 tc = cconj(t);
@@ -253,33 +250,29 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
         if constexpr (KIND == K_PLAIN) {
             // the gradient plane of this position (applied to the bin after the sweep)
             E.inv(NoOp(), staged<16>(
-                              [&](int y, int x) {
-                                  const int i = y * N + x;
-                                  return make_float4(ta[i].x, ta[i].y, ws[i].x, ws[i].y);
+                              [&](const Pt& p) {
+                                  const float2 tt = ta[p.e], ww = ws[p.e];
+                                  return make_float4(tt.x, tt.y, ww.x, ww.y);
                               },
-                              [&](int y, int x, float2 v, float4 o) {
+                              [&](const Pt& p, float2 v, float4 o) {
                                   float gr;
                                   const float2 r = adjoint(v, make_float2(o.x, o.y), make_float2(o.z, o.w), gr);
-                                  g[y * N + x] = gr;
+                                  g.put(p.e, gr);
                                   return r;
                               }));
         } else {
             // accumulated into this workgroup's slab
-            E.inv(NoOp(), staged<8>(
-                              [&](int y, int x) {
-                                  const int i = y * N + x;
-                                  return AdjOp{ta[i], ws[i], g[i]};
-                              },
-                              [&](int y, int x, float2 v, const AdjOp& o) {
-                                  float gr;
-                                  const float2 r = adjoint(v, o.t, o.w, gr);
-                                  g[y * N + x] = o.g + gr;
-                                  return r;
-                              }));
+            E.inv(NoOp(), staged<8>([&](const Pt& p) { return AdjOp{ta[p.e], ws[p.e], g[p.e]}; },
+                                    [&](const Pt& p, float2 v, const AdjOp& o) {
+                                        float gr;
+                                        const float2 r = adjoint(v, o.t, o.w, gr);
+                                        g.put(p.e, o.g + gr);
+                                        return r;
+                                    }));
         }
         if (a > 0)
-            E.fwd(NoOp(), staged<16>([&](int y, int x) { return pT[x * N + y]; },
-                                     [&](int, int, float2 v, float2 o) { return cmulc(v, o); }));
+            E.fwd(NoOp(), staged<16>([&](const Pt& p) { return pTb[p.e]; },
+                                     [&](const Pt&, float2 v, float2 o) { return cmulc(v, o); }));
         else E.fwd();
     }
     // the engine now holds fftn(back_wave_f) (Fourier domain, R state)
@@ -287,9 +280,9 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
     // ---- probe term in the Fourier domain: shift(bw, -d) = ifftn(fftn(bw) .* conj(Phi)); slab [x][y]
     if constexpr (KIND == K_BATCH) {
         if (want_probe) {
-            float2* ps = A.pslab + (size_t)slot * N2;
-            E.each_R(staged<16>([&](int y, int x) { return ps[x * N + y]; },
-                                [&](int y, int x, float2& v, float2 o) { ps[x * N + y] = cadd(o, cmulc(v, phase(y, x))); }));
+            const BufC ps(A.pslab + (size_t)slot * N2, N2);
+            E.each_R(staged<16>([&](const Pt& p) { return ps[p.e]; },
+                                [&](const Pt& p, float2& v, float2 o) { ps.put(p.e, cadd(o, cmulc(v, phase(p.y, p.x)))); }));
         }
     }
 
@@ -299,21 +292,22 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
     if (want_pos) {
         if (exact) {
             // Parseval: sum_x bw conj(-dpsi0/drow) = (1/N^2) sum_q B(q) conj(Psi0(q)) (-2 pi i g/N)
-            E.each_R(staged<16>([&](int y, int x) { return F0[x * N + y]; }, [&](int y, int x, float2& v, float2 o) {
+            E.each_R(staged<16>([&](const Pt& p) { return F0b[p.e]; }, [&](const Pt& p, float2& v, float2 o) {
                 const float2 c = cmulc(v, o);                  // B conj(Psi0)
                 // Re(c * (-i) * 2 pi g / N) = Im(c) * 2 pi g / N
-                dr += (double)c.y * shift_grid<N>(x);
-                dc += (double)c.y * shift_grid<N>(y);
+                dr += (double)c.y * shift_grid<N>(p.x);
+                dc += (double)c.y * shift_grid<N>(p.y);
             }));
             const double f = 2.0 * M_PI / (double)N / (double)N2;
             dr = block_sum<NT>(dr, red) * f;
             dc = block_sum<NT>(dc, red) * f;
         } else {
             // back_wave_f = ifftn(B); MATLAB gradient() of conj(psi0), one-sided at the edges
-            E.inv(NoOp(), [&](int, int, float2 v) { return cscale(v, inv_n2); });
+            E.inv(NoOp(), [&](const Pt&, float2 v) { return cscale(v, inv_n2); });
             have_bw = true;
             const float2* p0 = save;
-            E.each_C([&](int y, int x, float2& bw) {
+            E.each_C([&](const Pt& p, float2& bw) {
+                const int y = p.y, x = p.x;
                 // clamped neighbours: the edge rows / columns become the one-sided differences
                 const int xp = min(x + 1, N - 1), xm = max(x - 1, 0);
                 const int yp = min(y + 1, N - 1), ym = max(y - 1, 0);
@@ -336,19 +330,20 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
     } else {
         // ---- sequential updates (STEP03_BACK.m:84-92, Run_MSET_recon.m:155-157)
         if (want_probe) {
-            float2* probe = A.probe;
+            const BufC pr(A.probe, N2);
             const float s2 = A.s2;
-            auto upd = [&](int y, int x, float2 v) {
+            auto upd = staged<16>([&](const Pt& p) { return pr[p.e]; }, [&](const Pt& p, float2 v, float2 o) {
                 const float2 w = cscale(v, inv_n2);
-                probe[y * N + x] = csub(probe[y * N + x], cscale(w, s2));
+                pr.put(p.e, csub(o, cscale(w, s2)));
                 return w;
-            };
+            });
+            auto unshift = [&](const Pt& p, float2 v) { return cmulc(v, phase(p.y, p.x)); };
             if (have_bw) {
                 // MATLAB shifts back_wave_f: fftn(bw) .* conj(Phi) -> ifftn
-                E.fwd(NoOp(), [&](int y, int x, float2 v) { return cmulc(v, phase(y, x)); });
+                E.fwd(NoOp(), unshift);
                 E.inv(NoOp(), upd);
             } else {
-                E.inv([&](int y, int x, float2 v) { return cmulc(v, phase(y, x)); }, upd);
+                E.inv(unshift, upd);
             }
         }
         if (t == 0 && want_pos) {
@@ -439,17 +434,19 @@ __global__ __launch_bounds__(Cfg<N>::NT) void k_plane_fft(const float2* in, floa
     Eng E;
     E.init(s, tw);
     if (dir == 0) {
-        E.template fwd<true>([&](int y, int x, float2) { return in[y * N + x]; },
-                             [&](int y, int x, float2 v) {
-                                 out[x * N + y] = v;
+        const BufC ib(in, N * N), ob(out, N * N);           // in [y][x], out [x][y]
+        E.template fwd<true>([&](const Pt& p, float2) { return ib[p.e]; },
+                             [&](const Pt& p, float2 v) {
+                                 ob.put(p.e, v);
                                  return v;
                              });
     } else {
+        const BufC ib(in, N * N), ob(out, N * N);           // in [x][y], out [y][x]
         const float sc = alpha / (float)(N * N);
-        E.template inv<true>([&](int y, int x, float2) { return in[x * N + y]; },
-                             [&](int y, int x, float2 v) {
+        E.template inv<true>([&](const Pt& p, float2) { return ib[p.e]; },
+                             [&](const Pt& p, float2 v) {
                                  const float2 w = cscale(v, sc);
-                                 out[y * N + x] = beta == 0.f ? w : cadd(cscale(out[y * N + x], beta), w);
+                                 ob.put(p.e, beta == 0.f ? w : cadd(cscale(ob[p.e], beta), w));
                                  return v;
                              });
     }
