@@ -192,9 +192,12 @@ struct NoOp {
 // An element functor split in two phases: load(p) fetches the element's global operands, apply(p, v,
 // op) combines them with the element.  An engine that holds many elements per thread issues the loads
 // of G elements back to back before the first apply, so their memory latencies overlap instead of
-// adding up.  Called as an ordinary functor, it runs both phases for one element.
-template <int G_, class Ld, class Ap> struct Staged {
+// adding up; used as a transform's post functor, the loads of its first PF elements may be issued
+// before the transform's last pass, whose work then hides their latency.  Called as an ordinary
+// functor, it runs both phases for one element.
+template <int G_, class Ld, class Ap, int PF_ = 0> struct Staged {
     static constexpr int G = G_;
+    static constexpr int PF = PF_;   // elements whose loads an engine may issue ahead of the transform
     Ld ld;
     Ap ap;
     __device__ __forceinline__ auto load(const Pt& p) const { return ld(p); }
@@ -203,11 +206,13 @@ template <int G_, class Ld, class Ap> struct Staged {
     }
     template <class V> __device__ __forceinline__ auto operator()(const Pt& p, V&& v) const { return ap(p, v, ld(p)); }
 };
-template <int G, class Ld, class Ap> __device__ __forceinline__ Staged<G, Ld, Ap> staged(Ld ld, Ap ap) {
-    return Staged<G, Ld, Ap>{ld, ap};
+template <int G, int PF = 0, class Ld, class Ap> __device__ __forceinline__ Staged<G, Ld, Ap, PF> staged(Ld ld, Ap ap) {
+    return Staged<G, Ld, Ap, PF>{ld, ap};
 }
 template <class F> struct IsStaged { static constexpr bool value = false; };
-template <int G, class Ld, class Ap> struct IsStaged<Staged<G, Ld, Ap>> { static constexpr bool value = true; };
+template <int G, class Ld, class Ap, int PF> struct IsStaged<Staged<G, Ld, Ap, PF>> { static constexpr bool value = true; };
+template <class F> struct PrefetchOf { static constexpr int value = 0; };
+template <int G, class Ld, class Ap, int PF> struct PrefetchOf<Staged<G, Ld, Ap, PF>> { static constexpr int value = PF; };
 
 // ------------------------------------------------------------------ one Stockham stage
 // ROWS: transform along x (lines are rows y); otherwise along y (lines are columns x).

@@ -28,6 +28,7 @@
 // for arrays stored [x][y]).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <utility>
 #include "fft_lds.hpp"
 
 namespace mset {
@@ -152,35 +153,76 @@ template <> struct RegEngine<128> {
     // (line, xidx(i)), index (h N + line) + (32 (i & 3) + 4 (i >> 2)) N in a transposed [x][y] array.
     // MAP: the functor returns the new value (LOADG: it is handed zero instead of the current one);
     // otherwise it sees the element by reference.
+    template <bool COL> __device__ __forceinline__ Pt point(int i, unsigned vb) const {
+        if (COL) return Pt{h + 4 * i, line, Idx{vb, (unsigned)(4 * N * i)}};
+        return Pt{line, xidx(i), Idx{vb, (unsigned)((32 * (i & 3) + 4 * (i >> 2)) * N)}};
+    }
+    __device__ __forceinline__ unsigned vbase() const { return (unsigned)(h * N + line); }
+
+    // operands of the first PF elements of a staged functor, loaded ahead (see Staged)
+    template <class F, int PF = PrefetchOf<F>::value> struct Ahead {
+        decltype(std::declval<const F&>().load(Pt{})) op[PF > 0 ? PF : 1];
+    };
+    template <bool COL, class F, class AH> __device__ __forceinline__ void load_ahead(const F& f, AH& ah) {
+        constexpr int PF = PrefetchOf<F>::value;
+        if constexpr (PF > 0) {
+            const unsigned vb = vbase();
+#pragma unroll
+            for (int i = 0; i < PF; ++i) ah.op[i] = f.load(point<COL>(i, vb));
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+
     template <bool COL, bool MAP, bool LOADG, class F> __device__ __forceinline__ void run_elems(const F& f) {
-        const unsigned vb = (unsigned)(h * N + line);
-        auto pt = [&](int i) -> Pt {
-            if (COL) return Pt{h + 4 * i, line, Idx{vb, (unsigned)(4 * N * i)}};
-            return Pt{line, xidx(i), Idx{vb, (unsigned)((32 * (i & 3) + 4 * (i >> 2)) * N)}};
-        };
         if constexpr (IsStaged<F>::value) {
-            constexpr int G = F::G;
-            static_assert(E % G == 0, "group size must divide 32");
-#pragma unroll
-            for (int g0 = 0; g0 < E; g0 += G) {
-                decltype(f.load(pt(0))) op[G];
-#pragma unroll
-                for (int i = 0; i < G; ++i) op[i] = f.load(pt(g0 + i));
-                __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                for (int i = 0; i < G; ++i) {
-                    if constexpr (MAP) v[g0 + i] = f.apply(pt(g0 + i), LOADG ? make_float2(0.f, 0.f) : v[g0 + i], op[i]);
-                    else f.apply(pt(g0 + i), v[g0 + i], op[i]);
-                }
-                __builtin_amdgcn_sched_barrier(0);
-            }
+            Ahead<F, 0> none;
+            run_staged<COL, MAP, LOADG, 0>(f, none);
         } else {
+            const unsigned vb = vbase();
 #pragma unroll
             for (int i = 0; i < E; ++i) {
-                if constexpr (MAP) v[i] = f(pt(i), LOADG ? make_float2(0.f, 0.f) : v[i]);
-                else f(pt(i), v[i]);
+                if constexpr (MAP) v[i] = f(point<COL>(i, vb), LOADG ? make_float2(0.f, 0.f) : v[i]);
+                else f(point<COL>(i, vb), v[i]);
                 group_fence(i);
             }
+        }
+    }
+    // a staged functor over the 32 elements; the operands of the first PF are already in `ah`
+    template <bool COL, bool MAP, bool LOADG, int PF, class F, class AH>
+    __device__ __forceinline__ void run_staged(const F& f, const AH& ah) {
+        constexpr int G = F::G;
+        static_assert(E % G == 0 && PF % G == 0 && PF <= E, "group size must divide 32 and the prefetch");
+        const unsigned vb = vbase();
+#pragma unroll
+        for (int g0 = 0; g0 < E; g0 += G) {
+            decltype(f.load(Pt{})) op[G];
+            if (g0 >= PF) {
+#pragma unroll
+                for (int i = 0; i < G; ++i) op[i] = f.load(point<COL>(g0 + i, vb));
+                __builtin_amdgcn_sched_barrier(0);
+            } else {
+#pragma unroll
+                for (int i = 0; i < G; ++i) op[i] = ah.op[g0 + i];
+            }
+#pragma unroll
+            for (int i = 0; i < G; ++i) {
+                const Pt p = point<COL>(g0 + i, vb);
+                if constexpr (MAP) v[g0 + i] = f.apply(p, LOADG ? make_float2(0.f, 0.f) : v[g0 + i], op[i]);
+                else f.apply(p, v[g0 + i], op[i]);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    // the post functor of a transform: its first PF operands are loaded before the last pass
+    template <bool COL, class F, class PASS> __device__ __forceinline__ void last_pass_and_post(const F& f, PASS pass) {
+        if constexpr (IsStaged<F>::value && PrefetchOf<F>::value > 0) {
+            Ahead<F> ah;
+            load_ahead<COL>(f, ah);
+            pass();
+            run_staged<COL, true, false, PrefetchOf<F>::value>(f, ah);
+        } else {
+            pass();
+            run_elems<COL, true, false>(f);
         }
     }
 
@@ -190,8 +232,7 @@ template <> struct RegEngine<128> {
         run_elems<true, true, LOADG>(pre);
         pass_dit<false>();
         transpose_c_to_r();
-        pass_dit<false>();
-        run_elems<false, true, false>(post);
+        last_pass_and_post<false>(post, [&] { pass_dit<false>(); });
     }
 
     template <bool LOADG = false, class Pre = NoOp, class Post = NoOp>
@@ -200,8 +241,7 @@ template <> struct RegEngine<128> {
         run_elems<false, true, LOADG>(pre);
         pass_dif<true>();
         transpose_r_to_c();
-        pass_dif<true>();
-        run_elems<true, true, false>(post);
+        last_pass_and_post<true>(post, [&] { pass_dif<true>(); });
     }
 
     template <class F> __device__ __forceinline__ void each_R(const F& f) {

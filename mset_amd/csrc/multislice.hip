@@ -196,7 +196,7 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
     for (int a = 0; a < (A.sset ? 0 : L); ++a) {
         const BufC ta(A.trans + (size_t)a * N2, N2);
         E.fwd(staged<16>([&](const Pt& p) { return ta[p.e]; }, [&](const Pt&, float2 v, float2 o) { return cmul(v, o); }),
-              staged<16>([&](const Pt& p) { return pTb[p.e]; }, [&](const Pt&, float2 v, float2 o) { return cmul(v, o); }));
+              staged<16, 32>([&](const Pt& p) { return pTb[p.e]; }, [&](const Pt&, float2 v, float2 o) { return cmul(v, o); }));
         const BufC sv(save + (size_t)(a + 1) * N2, N2);
         E.inv(NoOp(), [&](const Pt& p, float2 v) {
             const float2 w = cscale(v, inv_n2);
@@ -210,7 +210,7 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
     {
         const BufF MT(A.sqrtM + (size_t)k * N2, N2);        // sqrt(M), [x][y]
         float* Iout = A.inten ? A.inten + (size_t)k * N2 : nullptr;
-        E.fwd(NoOp(), staged<16>(
+        E.fwd(NoOp(), staged<16, 16>(
                           [&](const Pt& p) {
                               const float m = MT[p.e];
                               if (KIND == K_FWD) return make_float3(m, 0.f, 0.f);
@@ -249,7 +249,7 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
         };
         if constexpr (KIND == K_PLAIN) {
             // the gradient plane of this position (applied to the bin after the sweep)
-            E.inv(NoOp(), staged<16>(
+            E.inv(NoOp(), staged<16, 16>(
                               [&](const Pt& p) {
                                   const float2 tt = ta[p.e], ww = ws[p.e];
                                   return make_float4(tt.x, tt.y, ww.x, ww.y);
@@ -262,7 +262,7 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
                               }));
         } else {
             // accumulated into this workgroup's slab
-            E.inv(NoOp(), staged<8>([&](const Pt& p) { return AdjOp{ta[p.e], ws[p.e], g[p.e]}; },
+            E.inv(NoOp(), staged<8, 16>([&](const Pt& p) { return AdjOp{ta[p.e], ws[p.e], g[p.e]}; },
                                     [&](const Pt& p, float2 v, const AdjOp& o) {
                                         float gr;
                                         const float2 r = adjoint(v, o.t, o.w, gr);
@@ -271,8 +271,8 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
                                     }));
         }
         if (a > 0)
-            E.fwd(NoOp(), staged<16>([&](const Pt& p) { return pTb[p.e]; },
-                                     [&](const Pt&, float2 v, float2 o) { return cmulc(v, o); }));
+            E.fwd(NoOp(), staged<16, 32>([&](const Pt& p) { return pTb[p.e]; },
+                                         [&](const Pt&, float2 v, float2 o) { return cmulc(v, o); }));
         else E.fwd();
     }
     // the engine now holds fftn(back_wave_f) (Fourier domain, R state)
