@@ -124,6 +124,13 @@ int mset_rotate(mset_ctx* ctx, const float* in, float* out, int nx, int ny, int 
 int mset_rotate_resident(mset_ctx* ctx, const double* vec1, const double* vec2, const double* vec3,
                          double phi, double theta, double psi, int inverse);
 
+/* ---- total-variation regulariser (TV_method.m:5-84) -------------------------------------------- */
+/* out = TV_method(in, lambda, n_iter) for a host volume float[nz][ny][nx] (in may equal out): Chambolle's
+ * projection for min_u 1/2 |u - in|^2 + lambda TV(u), periodic forward differences, tau = 1/12.
+ * tv_error (nullable) gets lambda * sum |grad u|, the error TV_method returns. */
+int mset_tv(mset_ctx* ctx, const float* in, float* out, int nx, int ny, int nz, float lambda, int n_iter,
+            double* tv_error);
+
 /* ---- forward model as a data generator -------------------------------------------------------- */
 /* |fftn(exit wave)|^2 of every position at the resident state, DC-centred (the layout of the measured
  * data): float[num_scan_pos][ny][nx].  The measured patterns uploaded with the state are not used. */
@@ -158,6 +165,10 @@ int mset_recon_iterate(mset_recon* r, int path, int batch, int use_graph, int po
 int mset_recon_error_pass(mset_recon* r, double* errs);
 /* rec float[nz][ny][nx]; probes interleaved complex [n_tilts][ny][nx]; scan lists concatenated by tilt */
 int mset_recon_get(mset_recon* r, float* rec, float* probes, float* xlists, float* ylists);
+/* TV step of the driver (Run_MSET_recon.m:266-269) on the resident rec: rec = sf_max * TV_method(rec /
+ * sf_max, lambda, n_iter) with sf_max = max(rec); tv_error (nullable) as for mset_tv.  EINVAL when the
+ * maximum is zero (MATLAB would fill the volume with NaN). */
+int mset_recon_tv(mset_recon* r, float lambda, int n_iter, double* tv_error);
 /* BLS rollback (Run_MSET_recon.m:113-115, 245-247) */
 int mset_recon_snapshot(mset_recon* r);
 int mset_recon_restore(mset_recon* r);

@@ -8,7 +8,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SOURCES = ['multislice.hip', 'elementwise.hip', 'rotate.hip', 'capi.hip']
+SOURCES = ['multislice.hip', 'elementwise.hip', 'rotate.hip', 'tv.hip', 'capi.hip']
 LIB = os.path.join(HERE, 'libmset_amd.so')
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 ARCH = os.environ.get('MSET_OFFLOAD_ARCH', 'gfx950')

@@ -85,7 +85,11 @@ struct Workspace {
     DevBuf<float2> save, trans, probeF, pslab, PG;
     DevBuf<float> gslab, G;
     RotWork rot;
-    ~Workspace() { rot_release(rot); }
+    TvWork tv;
+    ~Workspace() {
+        rot_release(rot);
+        tv_release(tv);
+    }
 };
 
 }  // namespace
@@ -680,6 +684,23 @@ int mset_rotate(mset_ctx* c, const float* in, float* out, int nx, int ny, int nz
     return MSET_OK;
 }
 
+int mset_tv(mset_ctx* c, const float* in, float* out, int nx, int ny, int nz, float lambda, int n_iter,
+            double* tv_error) {
+    CHK(check_ctx(c));
+    if (!in || !out) return fail(MSET_EINVAL, "null argument");
+    if (nx <= 0 || ny <= 0 || nz <= 0) return fail(MSET_EINVAL, "sizes must be positive");
+    if (!(lambda > 0.f) || n_iter < 0) return fail(MSET_EINVAL, "TV needs lambda > 0 and n_iter >= 0");
+    const long long nv = (long long)nx * ny * nz;
+    DevBuf<float> buf;
+    CHK(buf.ensure((size_t)nv, "TV volume"));
+    if (tv_ensure(c->ws->tv, nv) != 0) return fail(MSET_ENOMEM, "hipMalloc: TV work");
+    HIPCHK(hipMemcpyAsync(buf.p, in, sizeof(float) * nv, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(tv_run(c->ws->tv, buf.p, buf.p, nx, ny, nz, lambda, n_iter, tv_error, c->stream));
+    HIPCHK(hipMemcpyAsync(out, buf.p, sizeof(float) * nv, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MSET_OK;
+}
+
 int mset_rotate_resident(mset_ctx* c, const double* v1, const double* v2, const double* v3, double phi, double theta,
                          double psi, int inverse) {
     CHK(check_uploaded(c));
@@ -838,6 +859,24 @@ int mset_recon_error_pass(mset_recon* r, double* errs) {
         CHK(issue_fwd(c, nullptr));
         CHK(read_errors(c, 0, c->npos, &errs[j]));
     }
+    return MSET_OK;
+}
+
+// rec = sf_max * TV_method(rec / sf_max, lambda, n_iter), sf_max = max(rec(:)) (Run_MSET_recon.m:266-269)
+int mset_recon_tv(mset_recon* r, float lambda, int n_iter, double* tv_error) {
+    CHK(check_recon(r));
+    if (!(lambda > 0.f) || n_iter < 0) return fail(MSET_EINVAL, "TV needs lambda > 0 and n_iter >= 0");
+    const long long nv = (long long)r->n * r->n * r->nz;
+    if (tv_ensure(r->ws->tv, nv) != 0) return fail(MSET_ENOMEM, "hipMalloc: TV work");
+    float m = 0.f;
+    HIPCHK(volume_max(r->ws->tv, r->rec.p, nv, &m, r->stream));
+    if (!(m != 0.f) || !std::isfinite(m))
+        return fail(MSET_EINVAL, "TV normalises by max(rec): the maximum is zero or not finite");
+    drop_graph(r);
+    HIPCHK(volume_scale(r->rec.p, nv, m, 0, r->stream));
+    HIPCHK(tv_run(r->ws->tv, r->rec.p, r->rec.p, r->n, r->n, r->nz, lambda, n_iter, tv_error, r->stream));
+    HIPCHK(volume_scale(r->rec.p, nv, m, 1, r->stream));
+    HIPCHK(hipStreamSynchronize(r->stream));
     return MSET_OK;
 }
 

@@ -55,4 +55,19 @@ hipError_t launch_update_volume(float* V, const float* G, int n, int nz, int bin
 hipError_t launch_update_positions(float* x, float* y, const float* dpos, int npos, float s3, hipStream_t st);
 hipError_t launch_positivity(float* V, size_t n, hipStream_t st);
 
+// total-variation regulariser (tv.hip): dual field (3 volumes) + scratch volume, reduction partials
+struct TvWork {
+    float* P = nullptr;
+    double* part = nullptr;
+    long long nv = 0;
+};
+int tv_ensure(TvWork& w, long long nv);
+void tv_release(TvWork& w);
+// u = TV_method(f, lambda, n_iter) on device volumes [nz][ny][nx] (u may equal f); err (nullable) gets
+// lambda * TV(u)
+hipError_t tv_run(TvWork& w, const float* f, float* u, int nx, int ny, int nz, float lambda, int n_iter, double* err,
+                  hipStream_t st);
+hipError_t volume_max(TvWork& w, const float* v, long long n, float* out, hipStream_t st);
+hipError_t volume_scale(float* v, long long n, float s, int mode, hipStream_t st);
+
 }  // namespace mset

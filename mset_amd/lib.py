@@ -24,7 +24,7 @@ EXPORTS = ['mset_create', 'mset_destroy', 'mset_last_error', 'mset_stream', 'mse
            'mset_synchronize', 'mset_simulate', 'mset_recon_create', 'mset_recon_destroy', 'mset_recon_set_tilt',
            'mset_recon_set_steps', 'mset_recon_iterate', 'mset_recon_error_pass', 'mset_recon_get',
            'mset_recon_snapshot', 'mset_recon_restore', 'mset_recon_profile_enable', 'mset_recon_profile_read',
-           'mset_recon_stream']
+           'mset_recon_stream', 'mset_tv', 'mset_recon_tv']
 PATH_PLAIN, PATH_BATCHED = 0, 1
 
 
@@ -74,6 +74,7 @@ def load():
         'mset_recon_get': ([P, P, P, P, P], I), 'mset_recon_snapshot': ([P], I), 'mset_recon_restore': ([P], I),
         'mset_recon_profile_enable': ([P, I], I), 'mset_recon_profile_read': ([P, C.c_char_p, P, P], I),
         'mset_recon_stream': ([P], P),
+        'mset_tv': ([P, P, P, I, I, I, F, I, P], I), 'mset_recon_tv': ([P, F, I, P], I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -231,6 +232,15 @@ class Context:
         _chk(self._L.mset_rotate(self._h, _ptr(v), _ptr(out), nx, ny, nz, _ptr(Rm), float(rot_step_init)))
         return out
 
+    def tv(self, vol, lam, n_iter=300):
+        """TV_method(vol, lambda, n_iter) of a host volume [nz, ny, nx] -> (output, error)."""
+        v = _f32(vol)
+        out = np.empty_like(v)
+        nz, ny, nx = v.shape
+        e = np.zeros(1, np.float64)
+        _chk(self._L.mset_tv(self._h, _ptr(v), _ptr(out), nx, ny, nz, float(lam), int(n_iter), _ptr(e)))
+        return out, float(e[0])
+
     # ---------------------------------------------------------------- instrumentation
     def profile(self, enable=True):
         _chk(self._L.mset_profile_enable(self._h, int(bool(enable))))
@@ -329,6 +339,12 @@ class Recon:
         _chk(self._L.mset_recon_get(self._h, _ptr(rec), _ptr(probes), _ptr(xl), _ptr(yl)))
         split = np.cumsum(self.npos)[:-1]
         return rec, probes, np.split(xl, split), np.split(yl, split)
+
+    def tv(self, lam, n_iter=300):
+        """rec = max(rec) * TV_method(rec / max(rec), lambda, n_iter); returns the TV error."""
+        e = np.zeros(1, np.float64)
+        _chk(self._L.mset_recon_tv(self._h, float(lam), int(n_iter), _ptr(e)))
+        return float(e[0])
 
     def snapshot(self):
         _chk(self._L.mset_recon_snapshot(self._h))

@@ -20,6 +20,7 @@
 //   Func_rotate_3Dvol_ROTmat_FourierShear.m:14-133, SheerTransforms_for3Drot.m:16-49,
 //   calculate_S_matrices_for3DFourierShearRot_perm{,_backward}_ver1.m, MatrixQuaternionRot.m,
 //   My_obtain_quaternion_from_matrix.m        -> oracle_rotate
+//   TV_method.m:5-84                           -> tv_method
 // plus an "exact" gradient mode (the true adjoint of the forward model; the reference's
 // STEP03_BACK.m:14-15 pairs the back-wave of plane a with the wave *after* plane a, which is not
 // the gradient of its own loss) and the gradient-at-fixed-state sums used by the batched path.
@@ -593,6 +594,80 @@ template <typename T> void rotate(const T* in, T* out, int Nx, int Ny, int Nz, c
 }
 
 // ------------------------------------------------------------------------------------------------
+// Total-variation regulariser, TV_method.m:5-26 (3D): Chambolle's projection for
+// min_u 1/2 |u - f|^2 + lambda TV(u) with periodic forward differences.  Volume [z][y][x], MATLAB dims
+// (x, y, z) = (1, 2, 3); circshift(A, [1 0 0]) is A(x - 1), circshift(A, [-1 0 0]) is A(x + 1).
+// In MATLAB P starts as double zeros and becomes single after the first update (double op single is
+// single), so the float restatement runs in single throughout.
+// ------------------------------------------------------------------------------------------------
+struct Vol3 {
+    int nx, ny, nz;
+    size_t at(int x, int y, int z) const {
+        x = (x + nx) % nx; y = (y + ny) % ny; z = (z + nz) % nz;
+        return ((size_t)z * ny + y) * nx + x;
+    }
+};
+
+// TV_div.m part: P1 + P2 + P3 - P1(x-1) - P2(y-1) - P3(z-1)
+template <typename T> T tv_div(const Vol3& g, const T* P0, const T* P1, const T* P2, int x, int y, int z) {
+    const size_t i = g.at(x, y, z);
+    T d = P0[i] + P1[i] + P2[i];
+    d = d - P0[g.at(x - 1, y, z)];
+    d = d - P1[g.at(x, y - 1, z)];
+    d = d - P2[g.at(x, y, z - 1)];
+    return d;
+}
+
+template <typename T>
+double tv_method(const Vol3& g, const T* f, T* out, double lambda, int niter) {
+    const size_t nv = (size_t)g.nx * g.ny * g.nz;
+    std::vector<T> P0(nv, T(0)), P1(nv, T(0)), P2(nv, T(0)), D(nv);
+    const T tau = (T)(1.0 / 12.0), lam = (T)lambda;
+    for (int it = 0; it < niter; ++it) {
+        // temp_term = tau .* TV_del(TV_div(P) - input / lambda)
+#pragma omp parallel for schedule(static)
+        for (int z = 0; z < g.nz; ++z)
+            for (int y = 0; y < g.ny; ++y)
+                for (int x = 0; x < g.nx; ++x) {
+                    const size_t i = g.at(x, y, z);
+                    D[i] = tv_div<T>(g, P0.data(), P1.data(), P2.data(), x, y, z) - f[i] / lam;
+                }
+        // P = TV_project(P + temp_term)
+#pragma omp parallel for schedule(static)
+        for (int z = 0; z < g.nz; ++z)
+            for (int y = 0; y < g.ny; ++y)
+                for (int x = 0; x < g.nx; ++x) {
+                    const size_t i = g.at(x, y, z);
+                    const T q0 = P0[i] + tau * (D[g.at(x + 1, y, z)] - D[i]);
+                    const T q1 = P1[i] + tau * (D[g.at(x, y + 1, z)] - D[i]);
+                    const T q2 = P2[i] + tau * (D[g.at(x, y, z + 1)] - D[i]);
+                    const T den = std::sqrt(std::max(T(1), q0 * q0 + q1 * q1 + q2 * q2));
+                    P0[i] = q0 / den;
+                    P1[i] = q1 / den;
+                    P2[i] = q2 / den;
+                }
+    }
+    // output = input - lambda * TV_div(P); error = lambda * TV_error(output)
+    for (int z = 0; z < g.nz; ++z)
+        for (int y = 0; y < g.ny; ++y)
+            for (int x = 0; x < g.nx; ++x) {
+                const size_t i = g.at(x, y, z);
+                out[i] = f[i] - lam * tv_div<T>(g, P0.data(), P1.data(), P2.data(), x, y, z);
+            }
+    double e = 0.0;
+    for (int z = 0; z < g.nz; ++z)
+        for (int y = 0; y < g.ny; ++y)
+            for (int x = 0; x < g.nx; ++x) {
+                const size_t i = g.at(x, y, z);
+                const double d0 = (double)(out[g.at(x + 1, y, z)] - out[i]);
+                const double d1 = (double)(out[g.at(x, y + 1, z)] - out[i]);
+                const double d2 = (double)(out[g.at(x, y, z + 1)] - out[i]);
+                e += std::sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+            }
+    return lambda * e;
+}
+
+// ------------------------------------------------------------------------------------------------
 // Position loop (Run_MSET_recon.m:133-158) and error pass (207-222)
 // ------------------------------------------------------------------------------------------------
 template <typename T>
@@ -912,6 +987,14 @@ void oracle_rotate_d(const double* in, double* out, int nx, int ny, int nz, cons
 }
 void oracle_rotate_f(const float* in, float* out, int nx, int ny, int nz, const double* R, double rotStepInit) {
     rotate<float>(in, out, nx, ny, nz, R, rotStepInit);
+}
+
+// TV_method(input, lambda, Niter) for a 3D volume [z][y][x]; returns the TV error term
+double oracle_tv_d(const double* in, double* out, int nx, int ny, int nz, double lambda, int niter) {
+    return tv_method<double>(Vol3{nx, ny, nz}, in, out, lambda, niter);
+}
+double oracle_tv_f(const float* in, float* out, int nx, int ny, int nz, double lambda, int niter) {
+    return tv_method<float>(Vol3{nx, ny, nz}, in, out, lambda, niter);
 }
 
 }  // extern "C"

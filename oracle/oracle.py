@@ -50,6 +50,7 @@ def lib():
             'oracle_error_tilt_f': ([I, I, I, D, D, I, P, P, P, P, P, P, I, I, I, P, P], I),
             'oracle_grad_fixed_d': ([I, I, I, D, D, I, P, P, P, P, P, P, P, I, P, P, P, P], I),
             'oracle_loss_d': ([I, I, I, D, D, I, P, P, P, P, P, P, I], D),
+            'oracle_tv_d': ([P, P, I, I, I, D, I], D), 'oracle_tv_f': ([P, P, I, I, I, D, I], D),
             'oracle_rot_matrix': ([P, P, P, D, D, D, I, P], I),
             'oracle_shear_plan': ([P, D, P, P, P], None),
             'oracle_rotate_d': ([P, P, I, I, I, P, D], None),
@@ -267,6 +268,18 @@ def loss(V, probe_, prop, xlist, ylist, M, binning, sigma, dx, mode=MODE_REF):
     xl = _f64(xlist); yl = _f64(ylist)
     return lib().oracle_loss_d(n, nz, int(binning), sigma, dx, int(mode), _p(V), _p(_c128(probe_)), _p(_c128(prop)), _p(xl),
                                _p(yl), _p(_f32(M)), xl.shape[0])
+
+
+# ---------------------------------------------------------------- total variation
+def tv(vol, lam, n_iter=300, single=False):
+    """TV_method.m(input, lambda, Niter) on a volume v[z, y, x] -> (output, error)."""
+    if single:
+        v = _f32(vol); out = np.empty_like(v); fn = lib().oracle_tv_f
+    else:
+        v = _f64(vol); out = np.empty_like(v); fn = lib().oracle_tv_d
+    nz, ny, nx = v.shape
+    e = fn(_p(v), _p(out), nx, ny, nz, float(lam), int(n_iter))
+    return out, float(e)
 
 
 # ---------------------------------------------------------------- rotation

@@ -411,3 +411,48 @@ def test_rotation_of_gaussian_blob_matches_analytic(ang):
     R, _ = O.rot_matrix(ang)
     assert rel(O.rot_3dvol(blob(c0), ang), blob(R.T @ c0)) < 1e-6
     assert rel(O.rot_3dvol(blob(c0), ang, inverse=True), blob(R @ c0)) < 1e-6
+
+
+# ------------------------------------------------------------------ TV regulariser (TV_method.m)
+def test_tv_constant_volume_is_a_fixed_point():
+    f = np.full((6, 5, 7), 0.37)
+    out, err = O.tv(f, 0.1, 50)
+    assert np.array_equal(out, f) and err == 0.0
+
+
+def test_tv_preserves_the_mean_and_denoises():
+    rng = np.random.default_rng(9)
+    f = rng.uniform(0, 1, (8, 10, 12))
+    out, err = O.tv(f, 0.05, 100)
+    assert abs(out.sum() / f.sum() - 1) < 1e-12          # sum of a periodic divergence is zero
+    tv_of = lambda u: sum(np.sqrt((np.roll(u, -1, 2) - u) ** 2 + (np.roll(u, -1, 1) - u) ** 2
+                              + (np.roll(u, -1, 0) - u) ** 2).ravel())
+    assert tv_of(out) < 0.7 * tv_of(f)
+    assert abs(err / (0.05 * tv_of(out)) - 1) < 1e-12     # error = lambda * TV(output)
+
+
+def test_tv_periodic_step_converges_to_the_rof_solution():
+    """A step along z with two periodic jumps: the ROF minimiser keeps the jumps and moves each plateau of
+    length n/2 by 2 lambda/(n/2) towards the other (0.5 lambda^-1 |u - f|^2 + TV).  A wrong shift direction
+    or a div that is not the adjoint of the difference would not converge to it."""
+    nz, lam, a, b = 16, 0.4, 1.0, 0.0
+    f = np.zeros((nz, 3, 4))
+    f[: nz // 2] = a
+    f[nz // 2:] = b
+    out, err = O.tv(f, lam, 4000)
+    alpha, beta = a - 2 * lam / (nz / 2), b + 2 * lam / (nz / 2)
+    assert np.abs(out[: nz // 2] - alpha).max() < 1e-3 and np.abs(out[nz // 2:] - beta).max() < 1e-3
+    assert abs(err / (lam * 3 * 4 * 2 * (alpha - beta)) - 1) < 1e-3
+    # the same step along x and along y (axes are not interchanged)
+    for ax in (1, 2):
+        g = np.moveaxis(f, 0, ax)
+        o2, _ = O.tv(np.ascontiguousarray(g), lam, 4000)
+        assert np.abs(np.moveaxis(o2, ax, 0) - out).max() < 1e-3
+
+
+def test_tv_single_matches_double():
+    rng = np.random.default_rng(3)
+    f = rng.uniform(0, 1, (6, 8, 8))
+    od, ed = O.tv(f, 0.1, 300)
+    of, ef = O.tv(f, 0.1, 300, single=True)
+    assert rel(of, od) < 1e-5 and abs(ef / ed - 1) < 1e-5
