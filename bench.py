@@ -59,7 +59,8 @@ def main():
     ap.add_argument('--batch', type=int, default=1024)
     ap.add_argument('--no-graph', action='store_true')
     ap.add_argument('--plain-tilts', type=int, default=1, help='tilts of the plain path to time (0 = skip)')
-    ap.add_argument('--cpu-positions', type=int, default=256, help='positions of the CPU sample (0 = skip)')
+    ap.add_argument('--cpu-positions', type=int, default=1024,
+                    help='positions of the CPU sample, one full tilt by default (0 = skip)')
     ap.add_argument('--cpu-threads', type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
     world = int(os.environ.get('WORLD_SIZE', '1'))
