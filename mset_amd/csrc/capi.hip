@@ -82,7 +82,7 @@ using ProfMap = std::map<std::string, Prof>;
 
 // scratch shared by the tilts of one reconstruction
 struct Workspace {
-    DevBuf<float2> save, trans, probeF, pslab, PG;
+    DevBuf<float2> save, trans, probeF, pslab, PG, planes;
     DevBuf<float> gslab, G;
     RotWork rot;
     TvWork tv;
@@ -227,6 +227,7 @@ PosArgs base_args(mset_ctx* c) {
     A.gslab = c->ws->gslab.p;
     A.pslab = c->ws->pslab.p;
     A.dpos = c->dpos.p;
+    A.planes = plane_in_global(c->n) ? c->ws->planes.p : nullptr;
     return A;
 }
 
@@ -243,6 +244,8 @@ int ensure_fixed_buffers(mset_ctx* c) {
     Workspace& W = *c->ws;
     CHK(W.trans.ensure(plane(c) * c->L, "transmission"));
     CHK(W.probeF.ensure(plane(c), "probe spectrum"));
+    // the position kernels of the largest planes keep the plane in global memory: 2 planes per slot
+    if (plane_in_global(c->n)) CHK(W.planes.ensure((size_t)batch_slots(c) * 2 * plane(c), "plane buffers"));
     return MSET_OK;
 }
 
@@ -265,7 +268,7 @@ int prepare_fixed(mset_ctx* c) {
     CHK(ensure_fixed_buffers(c));
     HIPCHK(launch_transmission(c->V.p, c->ws->trans.p, c->n, c->nz, binning(c), sset(c) ? 1 : 0, c->p.sigma,
                                c->p.potential_pixelsize, c->stream));
-    HIPCHK(launch_plane_fft(c->n, c->probe.p, c->ws->probeF.p, c->tw.p, 0, 1.f, 0.f, c->stream));
+    HIPCHK(launch_plane_fft(c->n, c->probe.p, c->ws->probeF.p, c->tw.p, 0, 1.f, 0.f, c->ws->planes.p, c->stream));
     return MSET_OK;
 }
 
@@ -329,13 +332,14 @@ int issue_grad(mset_ctx* c, int first, int count, bool apply, int want) {
     if (want & WANT_PROBE) {
         // in place into slab 0 (each element is read and written by the same thread), then PG = ifftn
         HIPCHK(launch_reduce_cslabs(W.pslab.p, grid, plane(c), W.pslab.p, c->stream));
-        HIPCHK(launch_plane_fft(c->n, W.pslab.p, W.PG.p, c->tw.p, 1, 1.f, 0.f, c->stream));
+        HIPCHK(launch_plane_fft(c->n, W.pslab.p, W.PG.p, c->tw.p, 1, 1.f, 0.f, W.planes.p, c->stream));
     }
     if (apply) {
         if (c->p.step_size[0] != 0.f)
             HIPCHK(launch_update_volume(c->V.p, W.G.p, c->n, c->nz, binning(c), c->p.step_size[0], c->stream));
         if (c->p.step_size[1] != 0.f && (want & WANT_PROBE))
-            HIPCHK(launch_plane_fft(c->n, W.pslab.p, c->probe.p, c->tw.p, 1, -c->p.step_size[1], 1.f, c->stream));
+            HIPCHK(launch_plane_fft(c->n, W.pslab.p, c->probe.p, c->tw.p, 1, -c->p.step_size[1], 1.f, W.planes.p,
+                                    c->stream));
         if (c->p.step_size[2] != 0.f && (want & WANT_POS))
             HIPCHK(launch_update_positions(c->x.p + first, c->y.p + first, c->dpos.p + 2 * (size_t)first, count,
                                            c->p.step_size[2], c->stream));
@@ -403,7 +407,7 @@ int upload_impl(mset_ctx* c, const mset_params* p, const float* RVol, const floa
         return fail(MSET_EINVAL, "sizes must be positive");
     if (!plane_size_supported(p->nx))
         return fail(MSET_EUNSUPPORTED, "lateral size " + std::to_string(p->nx) +
-                                           " not built (supported: 16, 24, 32, 48, 64, 96, 128)");
+                                           " not built (supported: 16, 24, 32, 48, 64, 96, 128, 256)");
     if (p->slice_binning <= 0) return fail(MSET_EINVAL, "slice_binning must be a positive integer");
     if (p->grad_mode != MSET_GRAD_REF && p->grad_mode != MSET_GRAD_EXACT) return fail(MSET_EINVAL, "bad grad_mode");
     HIPCHK(hipStreamSynchronize(c->stream));

@@ -315,6 +315,7 @@ template <int N, int NT_> struct LdsEngine {
         s = lds;
         tw = lds_tw;
     }
+    __device__ __forceinline__ void bind(float2*) {}   // the plane lives in LDS
     template <bool LOADG = false, class Pre = NoOp, class Post = NoOp>
     __device__ __forceinline__ void fwd(const Pre& pre = Pre(), const Post& post = Post()) {
         fft2d_fwd<N, NT, LOADG>(s, tw, pre, post);

@@ -78,6 +78,7 @@ template <> struct RegEngine<128> {
         line = ((t >> 6) << 4) | (t & 15);
         h = (t >> 4) & 3;
     }
+    __device__ __forceinline__ void bind(float2*) {}   // the plane lives in registers
 
     static __device__ __forceinline__ int A(int y, int x) { return y * SP + x; }
     // element index along the line of register i in layout X (layout D: h + 4 i)

@@ -32,6 +32,7 @@ struct PosArgs {
     float2* save;                // per-slot scratch: (L + 2) planes
     float* gslab;                // per-slot object-gradient slab, L planes (K_PLAIN: slot 0, this position)
     float2* pslab;               // K_BATCH: per-slot probe-gradient plane (Fourier domain)
+    float2* planes;              // N = 256: per-slot plane buffers (2 N^2), the tiled engine's state
     float* dpos;                 // K_BATCH: per-position update terms (2 per position)
 };
 
@@ -41,7 +42,8 @@ hipError_t launch_plain(int n, const PosArgs& A, int k, hipStream_t st);
 hipError_t launch_batch(int n, const PosArgs& A, int grid, hipStream_t st);
 hipError_t launch_fwd(int n, const PosArgs& A, int grid, hipStream_t st);
 hipError_t launch_plane_fft(int n, const float2* in, float2* out, const float2* tw, int dir, float alpha, float beta,
-                            hipStream_t st);
+                            float2* planes, hipStream_t st);
+bool plane_in_global(int n);     // the position kernels keep the plane in global memory (planes buffer)
 hipError_t occupancy_batch(int n, int* blocks_per_cu);
 
 // element-wise kernels (elementwise.hip)

@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 #include "fft_lds.hpp"
 #include "fft_reg.hpp"
+#include "fft_tiled.hpp"
 #include "kernels.hpp"
 
 namespace mset {
@@ -42,6 +43,7 @@ template <> struct Cfg<48>  { static constexpr int NT = 192; using Eng = LdsEngi
 template <> struct Cfg<64>  { static constexpr int NT = 256; using Eng = LdsEngine<64, 256>; };
 template <> struct Cfg<96>  { static constexpr int NT = 576; using Eng = LdsEngine<96, 576>; };
 template <> struct Cfg<128> { static constexpr int NT = 512; using Eng = RegEngine<128>; };
+template <> struct Cfg<256> { static constexpr int NT = 512; using Eng = TiledEngine<256, 512>; };
 
 enum { K_PLAIN = 0, K_BATCH = 1, K_FWD = 2 };
 
@@ -94,6 +96,7 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
     const float inv_n2 = 1.0f / (float)N2;
     float2* save = A.save + (size_t)slot * (L + 2) * N2;   // planes 0..L: waves [y][x]; L+1: Fourier psi0 [x][y]
     float2* F0 = save + (size_t)(L + 1) * N2;
+    E.bind(A.planes ? A.planes + (size_t)slot * 2 * N2 : nullptr);
     const bool want_pos = (A.want & WANT_POS) != 0;
     const bool want_probe = (A.want & WANT_PROBE) != 0;
     const bool exact = A.grad_mode == GRAD_EXACT;
@@ -425,7 +428,7 @@ __global__ __launch_bounds__(Cfg<N>::NT) void k_fwd(PosArgs A) {
 //   dir 1: out = beta * out + alpha * ifftn(in) (with 1/N^2), in transposed ([x][y]), out row-major.
 template <int N>
 __global__ __launch_bounds__(Cfg<N>::NT) void k_plane_fft(const float2* in, float2* out, const float2* twg, int dir,
-                                                         float alpha, float beta) {
+                                                         float alpha, float beta, float2* planes) {
     constexpr int NT = Cfg<N>::NT;
     using Eng = typename Cfg<N>::Eng;
     __shared__ float2 s[Eng::LDS_ELEMS];
@@ -433,6 +436,7 @@ __global__ __launch_bounds__(Cfg<N>::NT) void k_plane_fft(const float2* in, floa
     load_twiddles<N, NT>(tw, twg);
     Eng E;
     E.init(s, tw);
+    E.bind(planes);
     if (dir == 0) {
         const BufC ib(in, N * N), ob(out, N * N);           // in [y][x], out [x][y]
         E.template fwd<true>([&](const Pt& p, float2) { return ib[p.e]; },
@@ -462,12 +466,15 @@ __global__ __launch_bounds__(Cfg<N>::NT) void k_plane_fft(const float2* in, floa
         case 64: { constexpr int NN = 64; BODY; } break;                    \
         case 96: { constexpr int NN = 96; BODY; } break;                    \
         case 128: { constexpr int NN = 128; BODY; } break;                  \
+        case 256: { constexpr int NN = 256; BODY; } break;                  \
         default: return hipErrorInvalidValue;                               \
     }
 
 bool plane_size_supported(int n) {
-    return n == 16 || n == 24 || n == 32 || n == 48 || n == 64 || n == 96 || n == 128;
+    return n == 16 || n == 24 || n == 32 || n == 48 || n == 64 || n == 96 || n == 128 || n == 256;
 }
+
+bool plane_in_global(int n) { return n == 256; }
 
 int position_threads(int n) {
     switch (n) {
@@ -478,6 +485,7 @@ int position_threads(int n) {
         case 64: return Cfg<64>::NT;
         case 96: return Cfg<96>::NT;
         case 128: return Cfg<128>::NT;
+        case 256: return Cfg<256>::NT;
         default: return 0;
     }
 }
@@ -498,8 +506,9 @@ hipError_t launch_fwd(int n, const PosArgs& A, int grid, hipStream_t st) {
 }
 
 hipError_t launch_plane_fft(int n, const float2* in, float2* out, const float2* tw, int dir, float alpha, float beta,
-                            hipStream_t st) {
-    MSET_DISPATCH(n, hipLaunchKernelGGL(k_plane_fft<NN>, dim3(1), dim3(Cfg<NN>::NT), 0, st, in, out, tw, dir, alpha, beta));
+                            float2* planes, hipStream_t st) {
+    MSET_DISPATCH(n, hipLaunchKernelGGL(k_plane_fft<NN>, dim3(1), dim3(Cfg<NN>::NT), 0, st, in, out, tw, dir, alpha, beta,
+                                        planes));
     return hipGetLastError();
 }
 

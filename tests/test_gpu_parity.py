@@ -88,7 +88,7 @@ def test_rotation_matches_saved_rotated_volume(mset, cuau):
 
 @pytest.mark.parametrize('shape,ang', [((24, 32, 32), (0.0, 37.0, 0.0)), ((64, 64, 64), (12.0, -50.0, 7.0)),
                                        ((48, 96, 96), (-30.0, 0.0, 0.0)), ((32, 128, 128), (0.0, 0.0, 22.0)),
-                                       ((96, 48, 64), (5.0, 60.0, -15.0))])
+                                       ((96, 48, 64), (5.0, 60.0, -15.0)), ((16, 256, 256), (0.0, 20.0, 0.0))])
 def test_rotation_vs_oracle(mset, shape, ang):
     rng = np.random.default_rng(sum(shape))
     v = rng.standard_normal(shape).astype(np.float32)
@@ -101,7 +101,8 @@ def test_rotation_vs_oracle(mset, shape, ang):
 @pytest.mark.parametrize('n,nz,binning,mode', [(24, 6, 2, O.MODE_REF), (32, 12, 3, O.MODE_EXACT),
                                                (64, 8, 4, O.MODE_REF), (128, 16, 4, O.MODE_EXACT),
                                                (128, 7, 2, O.MODE_REF), (48, 5, 2, O.MODE_EXACT),
-                                               (96, 4, 1, O.MODE_REF)])
+                                               (96, 4, 1, O.MODE_REF), (256, 8, 4, O.MODE_REF),
+                                               (256, 6, 2, O.MODE_EXACT)])
 def test_grad_at_fixed_state_vs_oracle(mset, n, nz, binning, mode):
     c = small_case(n=n, nz=nz, binning=binning, npos=5, seed=n + nz, spread=n / 8)
     print('case: grad', n, nz, binning, mode, flush=True)
@@ -123,7 +124,8 @@ def test_grad_at_fixed_state_vs_oracle(mset, n, nz, binning, mode):
 
 @pytest.mark.parametrize('n,nz,binning,mode', [(24, 6, 2, O.MODE_REF), (32, 9, 3, O.MODE_EXACT),
                                                (64, 8, 4, O.MODE_REF), (48, 7, 3, O.MODE_EXACT),
-                                               (128, 8, 4, O.MODE_REF), (128, 5, 2, O.MODE_EXACT)])
+                                               (128, 8, 4, O.MODE_REF), (128, 5, 2, O.MODE_EXACT),
+                                               (256, 4, 2, O.MODE_REF)])
 def test_sequential_run_vs_oracle(mset, n, nz, binning, mode):
     """'run_mset' semantics: per-position updates of V, probe and position (Run_MSET_recon.m:134-158)."""
     c = small_case(n=n, nz=nz, binning=binning, npos=6, seed=3 * n + nz, spread=n / 8)
