@@ -28,6 +28,13 @@ sys.path.insert(0, ROOT)
 METRIC = 'probe-position·slices/sec per fwd+adjoint iteration; HBM GB/s; p50/p99 latency'
 UNIT = 'probe-position·slices/s'
 HBM_PEAK_GBS = 8000.0
+WORKLOADS = {
+    'C2': 'C2: 64^3 volume at 0.4 A padded to 128x128x64, probe 128x128 300 kV 21 mrad, 32x32 scan '
+          '(2 px + U[-0.5,0.5]), 11 tilts -50..50 deg, 16 planes (1.6 A), one update pass over all tilts per step',
+    'C3': 'C3: 128^3 volume at 0.4 A padded to 256x256x128, probe 256x256 300 kV 21 mrad, 21 tilts -60..60 deg, '
+          'each tilt a square scan of side U{24..48} (2 px + U[-0.5,0.5]), 32 planes (1.6 A), one update pass '
+          'over all tilts per step',
+}
 
 
 def log(*a):
@@ -56,11 +63,14 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=5)
     ap.add_argument('--warmup', type=int, default=2)
-    ap.add_argument('--batch', type=int, default=1024)
+    ap.add_argument('--workload', default='C2', choices=['C2', 'C3'],
+                    help='C2 (default): the configuration the metric is quoted on; C3: 256^2 probe, uneven scans')
+    ap.add_argument('--batch', type=int, default=None, help='positions per mini-batch (default 1024 C2, 512 C3)')
     ap.add_argument('--no-graph', action='store_true')
-    ap.add_argument('--plain-tilts', type=int, default=1, help='tilts of the plain path to time (0 = skip)')
-    ap.add_argument('--cpu-positions', type=int, default=1024,
-                    help='positions of the CPU sample, one full tilt by default (0 = skip)')
+    ap.add_argument('--plain-tilts', type=int, default=None, help='tilts of the plain path to time (0 = skip; '
+                    'default 1 for C2, 0 for C3)')
+    ap.add_argument('--cpu-positions', type=int, default=None,
+                    help='positions of the CPU sample (0 = skip; default one full tilt for C2, 32 for C3)')
     ap.add_argument('--cpu-threads', type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
     world = int(os.environ.get('WORLD_SIZE', '1'))
@@ -70,23 +80,30 @@ def main():
         sys.exit(2)
 
     from mset_amd import lib, synth
+    big = args.workload == 'C3'
+    if args.batch is None:
+        args.batch = 512 if big else 1024
+    if args.plain_tilts is None:
+        args.plain_tilts = 0 if big else 1
+    if args.cpu_positions is None:
+        args.cpu_positions = 32 if big else 1024
 
     steps = (100.0, 0.0, 0.0)     # object step only, as the reference's example configuration (main.m:42)
-    log('generating workload C2 ...')
-    wl = synth.Workload('C2')
+    log('generating workload %s ...' % args.workload)
+    wl = synth.Workload(args.workload)
     ctx = lib.Context(0)
     wl.simulate(ctx)
     ctx.close()
     ntilt = len(wl.angles)
     rec0 = np.zeros_like(wl.truth)
-    mean_I = sum(float(m.sum()) for m in wl.measured) / ntilt / wl.npos
+    mean_I = sum(float(m.sum()) for m in wl.measured) / wl.positions()
     from mset_amd import physics
     probe0 = physics.normalise_probe(wl.probe.astype(np.complex128), mean_I).astype(np.complex64)
 
     def make_recon():
         r = lib.Recon(rec0, wl.prop, ntilt, wl.sigma, wl.dx, slice_binning=wl.binning, step_size=steps)
         for j, ang in enumerate(wl.angles):
-            r.set_tilt(j, ang, probe0, wl.xlists[j], wl.ylists[j], wl.measured[j], n_scan=wl.n_scan)
+            r.set_tilt(j, ang, probe0, wl.xlists[j], wl.ylists[j], wl.measured[j], n_scan=wl.n_scans[j])
         return r
 
     units = wl.pos_slices()
@@ -115,7 +132,7 @@ def main():
     b_alg = 32.0 * wl.P * wl.P            # bytes per probe-position-slice (store-exit-wave formulation)
     achieved = b_alg * per_launch_units / (avg_ms * 1e-3) / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, 'profiles', 'traffic_position_batch_c2.json')
+    tpath = os.path.join(ROOT, 'profiles', 'traffic_position_batch_%s.json' % args.workload.lower())
     if os.path.exists(tpath):
         with open(tpath) as fh:
             traffic = json.load(fh).get('bytes_per_launch')
@@ -125,14 +142,14 @@ def main():
         log('plain path: %d tilt(s)' % args.plain_tilts)
         rp = lib.Recon(rec0, wl.prop, args.plain_tilts, wl.sigma, wl.dx, slice_binning=wl.binning, step_size=steps)
         for j in range(args.plain_tilts):
-            rp.set_tilt(j, wl.angles[j], probe0, wl.xlists[j], wl.ylists[j], wl.measured[j], n_scan=wl.n_scan)
+            rp.set_tilt(j, wl.angles[j], probe0, wl.xlists[j], wl.ylists[j], wl.measured[j], n_scan=wl.n_scans[j])
         rp.profile(True)
         t0 = time.perf_counter()
         rp.iterate(lib.PATH_PLAIN, 1, use_graph=False, want_error=False)
         dtp = time.perf_counter() - t0
         npl, ms_pl = rp.profile_read('position_plain')
         rp.close()
-        pvalue = args.plain_tilts * wl.npos * wl.L / dtp
+        pvalue = sum(wl.npos_of(j) for j in range(args.plain_tilts)) * wl.L / dtp
         plain = dict(value=pvalue, unit=UNIT, tilts_timed=args.plain_tilts,
                      kernel_avg_ms=ms_pl / max(npl, 1), speedup_of_batched=value / pvalue)
 
@@ -145,10 +162,8 @@ def main():
         metric=METRIC, value=value, unit=UNIT, n_gpus=1, steps=args.steps, warmup=args.warmup,
         ms_per_step=ms_step, higher_is_better=True, scaling='weak', vs_baseline=None, dtype='f32 (complex64 waves)',
         data='synthetic: seeded FCC Gaussian-atom potential, device-simulated Poisson patterns',
-        config=dict(workload='C2: 64^3 volume at 0.4 A padded to 128x128x64, probe 128x128 300 kV 21 mrad, '
-                             '32x32 scan (2 px + U[-0.5,0.5]), 11 tilts -50..50 deg, 16 planes (1.6 A), '
-                             'one update pass over all tilts per step',
-                    path='batched', batch=args.batch, hip_graph=use_graph, positions_per_step=ntilt * wl.npos,
+        config=dict(workload=WORKLOADS[args.workload],
+                    path='batched', batch=args.batch, hip_graph=use_graph, positions_per_step=wl.positions(),
                     planes=wl.L, pos_slices_per_step=units, step_size=list(steps),
                     error_after=err_after, rotate_ms_per_step=ms_r),
         roofline=dict(bound='hbm', achieved=achieved, peak=HBM_PEAK_GBS, unit='GB/s', frac=achieved / HBM_PEAK_GBS,

@@ -35,15 +35,16 @@
 namespace mset {
 
 // plane engine and workgroup size per plane size: register-resident at 128, LDS stages elsewhere
-template <int N> struct Cfg;
-template <> struct Cfg<16>  { static constexpr int NT = 64;  using Eng = LdsEngine<16, 64>; };
-template <> struct Cfg<24>  { static constexpr int NT = 64;  using Eng = LdsEngine<24, 64>; };
-template <> struct Cfg<32>  { static constexpr int NT = 64;  using Eng = LdsEngine<32, 64>; };
-template <> struct Cfg<48>  { static constexpr int NT = 192; using Eng = LdsEngine<48, 192>; };
-template <> struct Cfg<64>  { static constexpr int NT = 256; using Eng = LdsEngine<64, 256>; };
-template <> struct Cfg<96>  { static constexpr int NT = 576; using Eng = LdsEngine<96, 576>; };
-template <> struct Cfg<128> { static constexpr int NT = 512; using Eng = RegEngine<128>; };
-template <> struct Cfg<256> { static constexpr int NT = 512; using Eng = TiledEngine<256, 512>; };
+template <int N> struct Cfg;   // NT threads; MINB: waves per SIMD the register budget must allow (HIP launch bounds)
+template <> struct Cfg<16> { static constexpr int NT = 64, MINB = 1;  using Eng = LdsEngine<16, 64>; };
+template <> struct Cfg<24> { static constexpr int NT = 64, MINB = 1;  using Eng = LdsEngine<24, 64>; };
+template <> struct Cfg<32> { static constexpr int NT = 64, MINB = 1;  using Eng = LdsEngine<32, 64>; };
+template <> struct Cfg<48> { static constexpr int NT = 192, MINB = 1; using Eng = LdsEngine<48, 192>; };
+template <> struct Cfg<64> { static constexpr int NT = 256, MINB = 1; using Eng = LdsEngine<64, 256>; };
+template <> struct Cfg<96> { static constexpr int NT = 576, MINB = 1; using Eng = LdsEngine<96, 576>; };
+template <> struct Cfg<128> { static constexpr int NT = 512, MINB = 1; using Eng = RegEngine<128>; };
+template <> struct Cfg<256> { static constexpr int NT = 512, MINB = 4; using Eng = TiledEngine<256, 512>; };
+
 
 enum { K_PLAIN = 0, K_BATCH = 1, K_FWD = 2 };
 
@@ -395,14 +396,14 @@ template <int N, int NT> __device__ __forceinline__ void load_twiddles(float2* t
     E.init(s, tw);
 
 template <int N>
-__global__ __launch_bounds__(Cfg<N>::NT) void k_plain(PosArgs A, int k) {
+__global__ __launch_bounds__(Cfg<N>::NT, Cfg<N>::MINB) void k_plain(PosArgs A, int k) {
     constexpr int NT = Cfg<N>::NT;
     MSET_LDS(N, NT)
     process_position<N, NT, K_PLAIN>(A, k, 0, E, phx, phy, red);
 }
 
 template <int N>
-__global__ __launch_bounds__(Cfg<N>::NT) void k_batch(PosArgs A) {
+__global__ __launch_bounds__(Cfg<N>::NT, Cfg<N>::MINB) void k_batch(PosArgs A) {
     constexpr int NT = Cfg<N>::NT;
     MSET_LDS(N, NT)
     for (int p = blockIdx.x; p < A.count; p += gridDim.x) {
@@ -413,7 +414,7 @@ __global__ __launch_bounds__(Cfg<N>::NT) void k_batch(PosArgs A) {
 }
 
 template <int N>
-__global__ __launch_bounds__(Cfg<N>::NT) void k_fwd(PosArgs A) {
+__global__ __launch_bounds__(Cfg<N>::NT, Cfg<N>::MINB) void k_fwd(PosArgs A) {
     constexpr int NT = Cfg<N>::NT;
     MSET_LDS(N, NT)
     for (int p = blockIdx.x; p < A.count; p += gridDim.x) {

@@ -21,6 +21,10 @@ CONFIGS = {
     # 11 tilts -50..50 deg, 1.6 A slices, 5e4 e/A^2
     'C2': dict(vol=64, P=128, dx=0.4, kV=300.0, alpha=21.0, defocus=0.0, scan=(32, 32), step_A=0.8, jitter=0.5,
                tilts=list(np.linspace(-50.0, 50.0, 11)), slice_A=1.6, dose=5e4, occupancy=0.08),
+    # batched: 128^3 at 0.4 A in a 256 x 256 field, 21 tilts -60..60 deg, each with its own square scan of
+    # side U{24..48} (2 px steps), 1.6 A slices (32 planes)
+    'C3': dict(vol=128, P=256, dx=0.4, kV=300.0, alpha=21.0, defocus=0.0, scan=(24, 48), step_A=0.8, jitter=0.5,
+               tilts=list(np.linspace(-60.0, 60.0, 21)), slice_A=1.6, dose=5e4, occupancy=0.08, uneven=True),
     # adjoint verification: 32^3 at 0.4 A in 64 x 64, 4 x 4 scan, 2 tilts, no binning
     'C4': dict(vol=32, P=64, dx=0.4, kV=300.0, alpha=21.0, defocus=0.0, scan=(4, 4), step_A=1.6, jitter=0.0,
                tilts=[-20.0, 20.0], slice_A=0.4, dose=1e5, occupancy=0.08),
@@ -84,14 +88,17 @@ class Workload:
         self.nz = self.truth.shape[0]
         self.L = -(-self.nz // self.binning)
         self.prop = physics.propagator(self.P, self.dx, self.lam, self.binning)
-        nsx, nsy = cfg['scan']
-        self.n_scan = (nsx, nsy)
         step_px = cfg['step_A'] / cfg['dx']
         cen = np.floor((self.P + 1) / 2 + 0.5)
-        origin = cen - (nsx - 1) * step_px / 2
         self.angles = [(0.0, float(t), 0.0) for t in cfg['tilts']]
-        self.xlists, self.ylists = [], []
+        self.xlists, self.ylists, self.n_scans = [], [], []
         for _ in self.angles:
+            if cfg.get('uneven'):
+                side = int(rng.integers(cfg['scan'][0], cfg['scan'][1] + 1))   # square grid, side U{lo..hi}
+                nsx = nsy = side
+            else:
+                nsx, nsy = cfg['scan']
+            origin = cen - (nsx - 1) * step_px / 2
             x, y = physics.scan_grid(nsx, nsy, step_px, origin)
             j = cfg['jitter']
             if j:
@@ -99,6 +106,8 @@ class Workload:
                 y = (y + rng.uniform(-j, j, y.shape)).astype(np.float32)
             self.xlists.append(x)
             self.ylists.append(y)
+            self.n_scans.append((nsx, nsy))
+        self.n_scan = self.n_scans[0]
         self.counts = cfg['dose'] * cfg['step_A'] ** 2
         self.probe = physics.normalise_probe(physics.probe(self.P, self.dx, self.lam, cfg['alpha'], cfg['defocus']),
                                              self.counts).astype(np.complex64)
@@ -107,18 +116,25 @@ class Workload:
 
     @property
     def npos(self):
+        """positions of the first tilt (every tilt of an even configuration)"""
         return self.n_scan[0] * self.n_scan[1]
 
+    def npos_of(self, j):
+        return self.n_scans[j][0] * self.n_scans[j][1]
+
+    def positions(self):
+        return sum(self.npos_of(j) for j in range(len(self.angles)))
+
     def pos_slices(self):
-        return len(self.angles) * self.npos * self.L
+        return self.positions() * self.L
 
     def simulate(self, ctx):
         """Poisson data for every tilt from the device forward model (one mset_ctx, reused)."""
         self.measured = []
-        zeros = np.zeros((self.npos, self.P, self.P), np.float32)
         for j, ang in enumerate(self.angles):
+            zeros = np.zeros((self.npos_of(j), self.P, self.P), np.float32)
             ctx.upload(self.truth, self.probe, self.prop, self.xlists[j], self.ylists[j], zeros, self.sigma, self.dx,
-                       slice_binning=self.binning, n_scan=self.n_scan)
+                       slice_binning=self.binning, n_scan=self.n_scans[j])
             ctx.rotate_resident(ang, inverse=True)
             I = ctx.simulate()
             self.measured.append(self.rng.poisson(np.maximum(I, 0).astype(np.float64)).astype(np.float32))
