@@ -22,6 +22,7 @@
 #ifndef MSET_H
 #define MSET_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -175,6 +176,13 @@ int mset_recon_restore(mset_recon* r);
 int mset_recon_profile_enable(mset_recon* r, int enable);
 int mset_recon_profile_read(mset_recon* r, const char* name, int* launches, double* total_ms);
 void* mset_recon_stream(mset_recon* r);
+
+/* ---- device capacity (admission control for several reconstructions on one device) ------------- */
+/* Free and total device memory in bytes. */
+int mset_device_memory(int device, size_t* free_bytes, size_t* total_bytes);
+/* Workgroups the batched position kernel keeps resident for plane size n: the number of per-slot
+ * workspaces (saved waves, gradient slabs) a context allocates. */
+int mset_batch_slots(int device, int n, int* slots);
 
 /* ---- instrumentation -------------------------------------------------------------------------- */
 /* When enabled, every launch of the named kernels is bracketed by HIP events on the context stream. */

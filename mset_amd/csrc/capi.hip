@@ -688,6 +688,31 @@ int mset_rotate(mset_ctx* c, const float* in, float* out, int nx, int ny, int nz
     return MSET_OK;
 }
 
+int mset_device_memory(int device, size_t* free_bytes, size_t* total_bytes) {
+    if (!free_bytes || !total_bytes) return fail(MSET_EINVAL, "null output");
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(MSET_EINVAL, "device index out of range");
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipMemGetInfo(free_bytes, total_bytes));
+    return MSET_OK;
+}
+
+int mset_batch_slots(int device, int n, int* slots) {
+    if (!slots) return fail(MSET_EINVAL, "null output");
+    if (!plane_size_supported(n)) return fail(MSET_EUNSUPPORTED, "plane size not built");
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(MSET_EINVAL, "device index out of range");
+    HIPCHK(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    int per_cu = 1;
+    HIPCHK(occupancy_batch(n, &per_cu));
+    *slots = prop.multiProcessorCount * (per_cu < 1 ? 1 : per_cu);
+    return MSET_OK;
+}
+
 int mset_tv(mset_ctx* c, const float* in, float* out, int nx, int ny, int nz, float lambda, int n_iter,
             double* tv_error) {
     CHK(check_ctx(c));

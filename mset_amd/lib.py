@@ -24,7 +24,7 @@ EXPORTS = ['mset_create', 'mset_destroy', 'mset_last_error', 'mset_stream', 'mse
            'mset_synchronize', 'mset_simulate', 'mset_recon_create', 'mset_recon_destroy', 'mset_recon_set_tilt',
            'mset_recon_set_steps', 'mset_recon_iterate', 'mset_recon_error_pass', 'mset_recon_get',
            'mset_recon_snapshot', 'mset_recon_restore', 'mset_recon_profile_enable', 'mset_recon_profile_read',
-           'mset_recon_stream', 'mset_tv', 'mset_recon_tv']
+           'mset_recon_stream', 'mset_tv', 'mset_recon_tv', 'mset_device_memory', 'mset_batch_slots']
 PATH_PLAIN, PATH_BATCHED = 0, 1
 
 
@@ -75,6 +75,7 @@ def load():
         'mset_recon_profile_enable': ([P, I], I), 'mset_recon_profile_read': ([P, C.c_char_p, P, P], I),
         'mset_recon_stream': ([P], P),
         'mset_tv': ([P, P, P, I, I, I, F, I, P], I), 'mset_recon_tv': ([P, F, I, P], I),
+        'mset_device_memory': ([I, P, P], I), 'mset_batch_slots': ([I, I, P], I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -82,6 +83,22 @@ def load():
         fn.restype = res
     _LIB = L
     return L
+
+
+def device_memory(device=0):
+    """(free, total) device memory in bytes."""
+    L = load()
+    f, t = C.c_size_t(0), C.c_size_t(0)
+    _chk(L.mset_device_memory(int(device), C.addressof(f), C.addressof(t)))
+    return int(f.value), int(t.value)
+
+
+def batch_slots(n, device=0):
+    """Resident workgroups of the batched position kernel for plane size n."""
+    L = load()
+    s = C.c_int(0)
+    _chk(L.mset_batch_slots(int(device), int(n), C.addressof(s)))
+    return int(s.value)
 
 
 def _ptr(a):

@@ -1,0 +1,99 @@
+"""Scheduling of the serving loop (mset_amd/serve.py) with stand-in reconstructions on the CPU: admission
+by memory budget, eviction of idle jobs and exact restore, one request per job at a time, results that do
+not depend on the mix."""
+import threading
+import time
+
+import numpy as np
+
+from mset_amd import serve
+
+
+class FakeRecon:
+    """State = (iterations done, probe marker); iterate sleeps a little to overlap with other requests."""
+    live = 0
+    peak = 0
+    lock = threading.Lock()
+
+    def __init__(self, spec, state=None):
+        self.spec = spec
+        self.n = 0 if state is None else int(state[0])
+        with FakeRecon.lock:
+            FakeRecon.live += 1
+            FakeRecon.peak = max(FakeRecon.peak, FakeRecon.live)
+
+    def iterate(self, path, batch, use_graph, positivity, want_error=True):
+        time.sleep(0.01)
+        self.n += 1
+        return 1.0 / (self.n + self.spec.job_id)
+
+    def get(self):
+        return (self.n, None, None, None)
+
+    def close(self):
+        with FakeRecon.lock:
+            FakeRecon.live -= 1
+
+
+def _specs(n):
+    return [serve.JobSpec(job_id=j, vol=32, P=64, tilts=[0.0, 10.0], scan=8, seed=j) for j in range(n)]
+
+
+def test_budget_limits_resident_jobs_and_eviction_restores_state():
+    FakeRecon.live = FakeRecon.peak = 0
+    specs = _specs(4)
+    one = serve.footprint_bytes(specs[0], 8)
+    srv = serve.Server(specs, workers=3, budget_bytes=int(2.5 * one), recon_factory=FakeRecon, slots_of=lambda n: 8)
+    order = [0, 1, 2, 3, 0, 1, 2, 3, 3, 0]
+    reqs = [srv.submit(j) for j in order]
+    for r in reqs:
+        assert r.done.wait(10)
+    srv.close()
+    assert all(not r.failure for r in reqs)
+    assert FakeRecon.peak <= 2                      # never more than the budget allows
+    assert srv.evictions >= 2
+    # each job's iterations ran in order and continued across evictions
+    for j in range(4):
+        errs = srv.jobs[j].errors
+        assert errs == [1.0 / (k + 1 + j) for k in range(order.count(j))]
+    rep = srv.report()
+    assert rep['requests'] == len(order) and rep['p99_ms'] >= rep['p50_ms'] > 0
+
+
+def test_job_larger_than_budget_is_rejected():
+    specs = _specs(1)
+    try:
+        serve.Server(specs, workers=1, budget_bytes=10, recon_factory=FakeRecon, slots_of=lambda n: 8)
+    except ValueError:
+        return
+    assert False, 'expected ValueError'
+
+
+def test_failed_request_releases_its_reservation():
+    class Broken(FakeRecon):
+        def iterate(self, *a, **k):
+            raise RuntimeError('device said no')
+
+    def factory(spec, state=None):
+        if spec.job_id == 0:
+            raise MemoryError('out of device memory')
+        return Broken(spec, state)
+
+    specs = _specs(2)
+    srv = serve.Server(specs, workers=2, budget_bytes=10 * serve.footprint_bytes(specs[0], 8), recon_factory=factory,
+                       slots_of=lambda n: 8)
+    a, b = srv.submit(0), srv.submit(1)
+    assert a.done.wait(10) and b.done.wait(10)
+    srv.close()
+    assert 'MemoryError' in a.failure and 'RuntimeError' in b.failure
+    assert srv.resident_bytes == srv.jobs[1].bytes      # job 1 stays resident, job 0 released its share
+
+
+def test_draw_jobs_respects_the_configuration():
+    specs = serve.draw_jobs(16, 3)
+    assert len(specs) == 16
+    for s in specs:
+        assert s.vol in serve.VOLUME_SIDES and s.P in serve.PROBE_SIDES and s.P >= s.vol
+        assert 8 <= len(s.tilts) <= 30 and 16 <= s.scan <= 40
+    t, jobs = serve.poisson_arrivals(200, 20.0, 16, 1)
+    assert abs(200 / t[-1] - 20.0) < 5.0 and np.all(np.diff(t) >= 0) and jobs.max() < 16
