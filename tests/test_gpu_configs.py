@@ -1,0 +1,157 @@
+"""GPU parity at the named configurations: C1 (the reference's CPU-runnable case, one update pass over all
+tilts), C3 (FP32 device gradient against the FP64 oracle at the 256^2, 32-plane shape) and C4 (device
+gradients against central finite differences of the FP64 loss along 50 seeded directions, h = 1e-4)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+
+def _workload(name, ctx, tilts=None):
+    from mset_amd import synth
+    wl = synth.Workload(name)
+    if tilts is not None:
+        wl.angles = [wl.angles[j] for j in tilts]
+        wl.xlists = [wl.xlists[j] for j in tilts]
+        wl.ylists = [wl.ylists[j] for j in tilts]
+        wl.n_scans = [wl.n_scans[j] for j in tilts]
+    wl.simulate(ctx)
+    return wl
+
+
+def test_c1_update_pass_plain_vs_oracle():
+    from mset_amd import lib
+    ctx = lib.Context(0)
+    wl = _workload('C1', ctx)
+    ctx.close()
+    steps = (100.0, 0.0, 0.0)
+    rec0 = (0.5 * wl.truth).astype(np.float32)
+    r = lib.Recon(rec0, wl.prop, len(wl.angles), wl.sigma, wl.dx, slice_binning=wl.binning, step_size=steps)
+    for j, ang in enumerate(wl.angles):
+        r.set_tilt(j, ang, wl.probe, wl.xlists[j], wl.ylists[j], wl.measured[j], n_scan=wl.n_scans[j])
+    e = r.iterate(lib.PATH_PLAIN, 1, want_error=True)
+    rec = r.get()[0]
+    r.close()
+    # the oracle composition of Run_MSET_recon.m:119-187
+    orec, errs = rec0.astype(np.float64), []
+    bprop = np.conj(wl.prop)
+    for j, ang in enumerate(wl.angles):
+        RV = O.rot_3dvol(orec, ang, inverse=True)
+        RV, _, _, _, ej, _ = O.run_tilt(RV, wl.probe, wl.prop, bprop, wl.xlists[j], wl.ylists[j], wl.measured[j],
+                                        *wl.n_scans[j], wl.binning, wl.sigma, wl.dx, steps)
+        errs.append(ej)
+        orec = O.rot_3dvol(RV, ang)
+        orec[orec < 0] = 0
+    print('C1 rec rel', rel(rec, orec), 'error', e, np.mean(errs), flush=True)
+    assert abs(e / np.mean(errs) - 1) < 1e-4
+    assert rel(rec - rec0, orec - rec0) < 1e-3
+
+
+def test_c3_fp32_gradient_vs_fp64_oracle():
+    from mset_amd import lib
+    ctx = lib.Context(0)
+    wl = _workload('C3', ctx, tilts=[3])
+    k = np.arange(0, wl.npos_of(0), max(1, wl.npos_of(0) // 4))[:4]
+    V = wl.truth
+    xl, yl, M = wl.xlists[0][k], wl.ylists[0][k], wl.measured[0][k]
+    ctx.upload(V, wl.probe, wl.prop, xl, yl, M, wl.sigma, wl.dx, slice_binning=wl.binning, n_scan=(len(k), 1))
+    dV, dp, dpos, err = ctx.grad()
+    w = ctx.waves(0)
+    ctx.close()
+    odV, odp, odpos, oep = O.grad_fixed(V, wl.probe, wl.prop, np.conj(wl.prop), xl, yl, M, wl.binning, wl.sigma, wl.dx)
+    psi0 = O.subpixel_shift(wl.probe, xl[0], yl[0])
+    save, _, _, _ = O.forward(V, psi0, wl.prop, M[0], wl.binning, wl.sigma, wl.dx)
+    r = dict(exit_wave=rel(w[-1], save[-1]), loss=abs(err / (oep.sum() / len(k)) - 1), dV=rel(dV, odV),
+             dprobe=rel(dp, odp), dpos=rel(dpos, odpos))
+    print('C3 FP32 vs FP64 relative errors', r, flush=True)
+    assert r['exit_wave'] < 1e-4 and r['loss'] < 1e-4
+    assert r['dV'] < 1e-3 and r['dprobe'] < 1e-3
+    # The position terms sum Re(grad(conj psi0) . bw) over 65536 pixels and cancel almost completely on this
+    # weak-phase state, so their relative error measures cancellation, not accuracy.  Bound the error by the
+    # summands' magnitude instead (MATLAB gradient() is numpy's edge_order=1 gradient).
+    for i in range(len(k)):
+        p0 = O.subpixel_shift(wl.probe, xl[i], yl[i])
+        sv, _, resi, _ = O.forward(V, p0, wl.prop, M[i], wl.binning, wl.sigma, wl.dx)
+        _, bw, _ = O.back(V, sv, p0, resi, np.conj(wl.prop), wl.binning, wl.sigma, wl.dx, xl[i], yl[i], O.MODE_REF)
+        gy, gx = np.gradient(np.conj(p0))
+        for ax, gg in ((0, gx), (1, gy)):
+            scale = np.sum(np.abs(gg) * np.abs(bw))
+            assert abs(dpos[i, ax] - odpos[i, ax]) <= 1e-4 * scale, (i, ax, dpos[i, ax], odpos[i, ax], scale)
+
+
+def test_c4_gradients_vs_central_differences():
+    """Exact-mode device gradients along 50 seeded directions against (L(x + h d) - L(x - h d)) / 2h of
+    the FP64 loss, whose own derivatives the oracle tests pin.  The reference-mode direction
+    (STEP03_BACK) is compared with the oracle's restatement of it."""
+    from mset_amd import lib
+    ctx = lib.Context(0)
+    wl = _workload('C4', ctx)
+    V = (0.5 * wl.truth + 0.25 * wl.truth.max()).astype(np.float64)
+    xl, yl, M = wl.xlists[0], wl.ylists[0], wl.measured[0]
+    n = wl.P
+    ctx.upload(V, wl.probe, wl.prop, xl, yl, M, wl.sigma, wl.dx, slice_binning=wl.binning, n_scan=wl.n_scans[0],
+               grad_mode=lib.GRAD_EXACT)
+    dV, dp, dpos, _ = ctx.grad()
+    loss = lambda V_=V, p_=wl.probe, x_=xl, y_=yl: O.loss(V_, p_, wl.prop, np.float64(x_), np.float64(y_), M,
+                                                        wl.binning, wl.sigma, wl.dx)
+    og = O.grad_fixed(V, wl.probe, wl.prop, np.conj(wl.prop), xl, yl, M, wl.binning, wl.sigma, wl.dx,
+                      mode=O.MODE_EXACT)
+    rng = np.random.default_rng(4)
+    h = 1e-4
+    c = -2 * n * n * wl.sigma * wl.dx
+    g = np.repeat(dV, wl.binning, axis=0)[:V.shape[0]].astype(np.float64)
+    go = np.repeat(og[0], wl.binning, axis=0)[:V.shape[0]]
+    fd, an, ao = [], [], []
+    for _ in range(50):
+        d = rng.standard_normal(V.shape)
+        fd.append((loss(V_=V + h * d) - loss(V_=V - h * d)) / (2 * h))
+        an.append(c * np.sum(g * d))
+        ao.append(c * np.sum(go * d))
+    fd, an, ao = map(np.array, (fd, an, ao))
+    # the FD is exact to FP64 precision here (checked against the FP64 adjoint); the device gradient carries
+    # FP32 rounding, so single directional derivatives that nearly cancel are compared as one 50-vector
+    print('C4: FD vs FP64 adjoint %.2e, FD vs device (FP32) %.2e over 50 directions'
+          % (np.linalg.norm(fd - ao) / np.linalg.norm(fd), np.linalg.norm(fd - an) / np.linalg.norm(fd)), flush=True)
+    assert np.linalg.norm(fd - ao) <= 1e-6 * np.linalg.norm(fd)
+    assert np.linalg.norm(fd - an) <= 1e-3 * np.linalg.norm(fd)
+    # probe (complex directions): the probe carries the dose (|probe| ~ sqrt(counts)), so the loss curves
+    # strongly along it and the central difference needs h ~ 1e-6 max|probe| to be exact to ~1e-7
+    odp, odpos = og[1], og[2]
+    hp = 1e-6 * float(np.abs(wl.probe).max())
+    fp, ap, aq = [], [], []
+    for _ in range(10):
+        d = rng.standard_normal(dp.shape) + 1j * rng.standard_normal(dp.shape)
+        fp.append((loss(p_=wl.probe + hp * d) - loss(p_=wl.probe - hp * d)) / (2 * hp))
+        ap.append(2 * n * n * np.real(np.sum(np.conj(dp.astype(np.complex128)) * d)))
+        aq.append(2 * n * n * np.real(np.sum(np.conj(odp) * d)))
+    fp, ap, aq = np.array(fp), np.array(ap), np.array(aq)
+    print('C4 probe: FD vs FP64 adjoint %.2e, FD vs device %.2e'
+          % (np.linalg.norm(fp - aq) / np.linalg.norm(fp), np.linalg.norm(fp - ap) / np.linalg.norm(fp)), flush=True)
+    assert np.linalg.norm(fp - aq) <= 1e-5 * np.linalg.norm(fp)
+    assert np.linalg.norm(fp - ap) <= 1e-3 * np.linalg.norm(fp), (fp, ap)
+    # positions: dL/dx_k = -2 n^2 dpos[k, 0] (the spectral position term); the FD with h = 1e-3 px is
+    # exact to ~1e-4 against the FP64 term, and the FP32 device term is bounded by its summands as in C3
+    hx = 1e-3
+    for kk in range(3):
+        xp, xm = np.float64(xl).copy(), np.float64(xl).copy()
+        xp[kk] += hx
+        xm[kk] -= hx
+        fd = (loss(x_=xp) - loss(x_=xm)) / (2 * hx)
+        assert abs(fd + 2 * n * n * odpos[kk, 0]) <= 1e-3 * abs(fd), (fd, odpos[kk, 0])
+        p0 = O.subpixel_shift(wl.probe, xl[kk], yl[kk])
+        sv, _, resi, _ = O.forward(V, p0, wl.prop, M[kk], wl.binning, wl.sigma, wl.dx)
+        _, bw, _ = O.back(V, sv, p0, resi, np.conj(wl.prop), wl.binning, wl.sigma, wl.dx, xl[kk], yl[kk],
+                          O.MODE_EXACT)
+        scale = np.sum(np.abs(np.gradient(np.conj(p0))[1]) * np.abs(bw))
+        assert abs(dpos[kk, 0] - odpos[kk, 0]) <= 1e-4 * scale, (kk, dpos[kk, 0], odpos[kk, 0], scale)
+    # the reference (STEP03_BACK) direction against its oracle restatement
+    ctx.upload(V, wl.probe, wl.prop, xl, yl, M, wl.sigma, wl.dx, slice_binning=wl.binning, n_scan=wl.n_scans[0],
+               grad_mode=lib.GRAD_REF)
+    rdV, rdp, rdpos, _ = ctx.grad()
+    ctx.close()
+    odV, odp, odpos, _ = O.grad_fixed(V, wl.probe, wl.prop, np.conj(wl.prop), xl, yl, M, wl.binning, wl.sigma,
+                                      wl.dx, mode=O.MODE_REF)
+    assert rel(rdV, odV) < 1e-3 and rel(rdp, odp) < 1e-3 and rel(rdpos, odpos) < 1e-3

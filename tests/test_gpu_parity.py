@@ -101,7 +101,9 @@ def test_rotation_vs_oracle(mset, shape, ang):
 @pytest.mark.parametrize('n,nz,binning,mode', [(24, 6, 2, O.MODE_REF), (32, 12, 3, O.MODE_EXACT),
                                                (64, 8, 4, O.MODE_REF), (128, 16, 4, O.MODE_EXACT),
                                                (128, 7, 2, O.MODE_REF), (48, 5, 2, O.MODE_EXACT),
-                                               (96, 4, 1, O.MODE_REF), (256, 8, 4, O.MODE_REF),
+                                               (96, 4, 1, O.MODE_REF), (64, 32, 1, O.MODE_EXACT),
+                                               (64, 6, 2, O.MODE_EXACT), (16, 4, 1, O.MODE_EXACT),
+                                               (256, 8, 4, O.MODE_REF),
                                                (256, 6, 2, O.MODE_EXACT)])
 def test_grad_at_fixed_state_vs_oracle(mset, n, nz, binning, mode):
     c = small_case(n=n, nz=nz, binning=binning, npos=5, seed=n + nz, spread=n / 8)
