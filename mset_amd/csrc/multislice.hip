@@ -165,16 +165,18 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
     const BufC F0b(F0, N2);                                 // Fourier psi0, [x][y]
     const BufC tr0(A.trans, N2);                            // transmission of plane 0, [y][x]
 
-    // ---- incident wave psi0 = ifftn(fftn(probe) .* Phi)
-    auto store_psi0 = [&](const Pt& p, float2 v) {
+    // ---- incident wave psi0 = ifftn(fftn(probe) .* Phi), stored, times t_0.  Each inverse transform of
+    // the forward sweep ends with the next plane's transmission, whose operands are loaded ahead of the
+    // last pass: the loads are then issued before the wave's stores and never wait behind them.
+    const bool multi = !A.sset && L > 0;
+    auto psi0_post = staged<16, 32>([&](const Pt& p) { return tr0[p.e]; }, [&](const Pt& p, float2 v, float2 o) {
         float2 w = cscale(v, inv_n2);
         if (KIND != K_FWD) sv0.put(p.e, w);
-        if (A.sset) {                                       // single slice: exit wave psi0 .* t, no propagation
-            w = cmul(w, tr0[p.e]);
-            if (KIND != K_FWD) sv0.put(Idx{p.e.v, p.e.s + (unsigned)N2}, w);
-        }
+        w = cmul(w, o);
+        // single slice: the exit wave is psi0 .* t, no propagation
+        if (A.sset && KIND != K_FWD) sv0.put(Idx{p.e.v, p.e.s + (unsigned)N2}, w);
         return w;
-    };
+    });
     if constexpr (KIND == K_PLAIN) {
         const BufC pr(A.probe, N2);
         E.template fwd<true>(staged<16>([&](const Pt& p) { return pr[p.e]; },
@@ -184,7 +186,7 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
                                  F0b.put(p.e, f);
                                  return f;
                              });
-        E.inv(NoOp(), store_psi0);
+        E.inv(NoOp(), psi0_post);
     } else {
         const BufC pf(A.probeF, N2);                        // fftn(probe), [x][y]
         E.template inv<true>(staged<16>([&](const Pt& p) { return pf[p.e]; },
@@ -193,20 +195,22 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
                                             if (KIND == K_BATCH) F0b.put(p.e, f);
                                             return f;
                                         }),
-                             store_psi0);
+                             psi0_post);
     }
 
-    // ---- forward sweep: psi_a = ifftn(fftn(psi_{a-1} .* t_a) .* p)
-    for (int a = 0; a < (A.sset ? 0 : L); ++a) {
-        const BufC ta(A.trans + (size_t)a * N2, N2);
-        E.fwd(staged<16>([&](const Pt& p) { return ta[p.e]; }, [&](const Pt&, float2 v, float2 o) { return cmul(v, o); }),
-              staged<16, 32>([&](const Pt& p) { return pTb[p.e]; }, [&](const Pt&, float2 v, float2 o) { return cmul(v, o); }));
+    // ---- forward sweep: psi_a = ifftn(fftn(psi_{a-1} .* t_a) .* p); the engine enters plane a holding
+    // psi_{a-1} .* t_a
+    for (int a = 0; a < (multi ? L : 0); ++a) {
+        E.fwd(NoOp(), staged<16, 32>([&](const Pt& p) { return pTb[p.e]; }, [&](const Pt&, float2 v, float2 o) { return cmul(v, o); }));
         const BufC sv(save + (size_t)(a + 1) * N2, N2);
-        E.inv(NoOp(), [&](const Pt& p, float2 v) {
+        const bool next = a + 1 < L;
+        // the last plane has no successor: an empty view reads zeros and the product is skipped
+        const BufC tn(A.trans + (size_t)(next ? a + 1 : 0) * N2, next ? N2 : 0u);
+        E.inv(NoOp(), staged<16, 32>([&](const Pt& p) { return tn[p.e]; }, [&](const Pt& p, float2 v, float2 o) {
             const float2 w = cscale(v, inv_n2);
             if (KIND != K_FWD) sv.put(p.e, w);
-            return w;
-        });
+            return next ? cmul(w, o) : w;
+        }));
     }
 
     // ---- exit wave, intensity, amplitude residual (STEP02_FORWARD_4D.m:36-49), r .* conj(p) fused
