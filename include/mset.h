@@ -108,6 +108,19 @@ int mset_grad(mset_ctx* ctx, int first, int count, float* dV, float* dprobe, flo
  * tmp_probe_wave and save_wave_f of STEP02_FORWARD_4D.m:27-35: interleaved complex [L + 1][ny][nx]. */
 int mset_waves(mset_ctx* ctx, int k, float* waves_out);
 
+/* Exit waves of the batched path.  STORE keeps every plane's wave for the adjoint (L + 2 planes per
+ * resident position, the reference's save_wave3D); RECOMPUTE keeps 5 and rebuilds psi_a in the adjoint by
+ * inverting the forward sweep, psi_a = conj(t_a) .* ifftn(fftn(psi_{a+1}) .* conj(prop)), for 50 % more
+ * transforms.  It needs a unit-modulus propagator (the Fresnel propagator of Func_generate_propagator.m
+ * is one; EINVAL otherwise).  AUTO (the default) stores unless the store scratch (waves and gradient
+ * slabs) would exceed a quarter of the device memory.  No counterpart in the reference, which stores.
+ * mset_exit_waves reports the variant and the per-position scratch bytes of the last batched call. */
+#define MSET_EXIT_WAVES_AUTO 0
+#define MSET_EXIT_WAVES_STORE 1
+#define MSET_EXIT_WAVES_RECOMPUTE 2
+int mset_set_exit_waves(mset_ctx* ctx, int mode);
+int mset_exit_waves(mset_ctx* ctx, int* resolved, size_t* bytes_per_slot);
+
 /* Mini-batch variant of 'run_mset': positions are taken in order in batches of `batch`; each batch's
  * gradient is formed at the state left by the previous batch and then applied.  batch == 1 equals
  * the sequential semantics up to summation order; step sizes 0 make it a forward + gradient pass. */
@@ -170,6 +183,8 @@ int mset_recon_get(mset_recon* r, float* rec, float* probes, float* xlists, floa
  * sf_max, lambda, n_iter) with sf_max = max(rec); tv_error (nullable) as for mset_tv.  EINVAL when the
  * maximum is zero (MATLAB would fill the volume with NaN). */
 int mset_recon_tv(mset_recon* r, float lambda, int n_iter, double* tv_error);
+/* exit-wave variant of the tilt series' batched passes (MSET_EXIT_WAVES_*, as mset_set_exit_waves) */
+int mset_recon_set_exit_waves(mset_recon* r, int mode);
 /* BLS rollback (Run_MSET_recon.m:113-115, 245-247) */
 int mset_recon_snapshot(mset_recon* r);
 int mset_recon_restore(mset_recon* r);

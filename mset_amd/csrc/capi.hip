@@ -84,6 +84,9 @@ using ProfMap = std::map<std::string, Prof>;
 struct Workspace {
     DevBuf<float2> save, trans, probeF, pslab, PG, planes;
     DevBuf<float> gslab, G;
+    int exit_waves = MSET_EXIT_WAVES_AUTO;   // requested exit-wave variant of the batched kernels
+    bool recompute = false;                  // the variant of the last batched call
+    size_t slot_bytes = 0;                   // its per-slot scratch (waves + gradient slab)
     RotWork rot;
     TvWork tv;
     ~Workspace() {
@@ -103,6 +106,7 @@ struct mset_ctx {
     int n = 0, nz = 0, L = 0, npos = 0;
     int method = MSET_METHOD_MSET;   // operator of the current call (MSET, or SSET = one plane of depth nz)
     bool uploaded = false;
+    bool prop_unitary = false;       // |prop| = 1 everywhere (to 1e-5): the forward sweep can be inverted
     DevBuf<float> V, sqrtM, x, y, dpos, Mtmp;
     DevBuf<float2> probe, prop, tw;
     DevBuf<double> err;
@@ -224,6 +228,7 @@ PosArgs base_args(mset_ctx* c) {
     A.ylist = c->y.p;
     A.err_pos = c->err.p;
     A.save = c->ws->save.p;
+    A.recompute = c->ws->recompute ? 1 : 0;
     A.gslab = c->ws->gslab.p;
     A.pslab = c->ws->pslab.p;
     A.dpos = c->dpos.p;
@@ -249,11 +254,35 @@ int ensure_fixed_buffers(mset_ctx* c) {
     return MSET_OK;
 }
 
+// exit-wave variant of a batched call (see mset_set_exit_waves): recompute only where it saves memory
+// (more than RECOMPUTE_PLANES planes stored) and the propagator is unit-modulus
+int resolve_exit_waves(mset_ctx* c, int slots, bool* recompute) {
+    Workspace& W = *c->ws;
+    const int L = planes(c);
+    const bool useful = !sset(c) && L + 2 > RECOMPUTE_PLANES;
+    *recompute = false;
+    if (W.exit_waves == MSET_EXIT_WAVES_RECOMPUTE) {
+        if (!c->prop_unitary) return fail(MSET_EINVAL, "recomputed exit waves need a unit-modulus propagator");
+        *recompute = useful;
+    } else if (W.exit_waves == MSET_EXIT_WAVES_AUTO && useful && c->prop_unitary) {
+        size_t fr = 0, total = 0;
+        HIPCHK(hipMemGetInfo(&fr, &total));
+        const double store = (double)slots * (double)plane(c) * ((L + 2) * sizeof(float2) + L * sizeof(float));
+        *recompute = store > 0.25 * (double)total;
+    }
+    return MSET_OK;
+}
+
 int ensure_batch_buffers(mset_ctx* c) {
     Workspace& W = *c->ws;
     const int s = batch_slots(c);
     CHK(ensure_fixed_buffers(c));
-    CHK(W.save.ensure((size_t)s * (c->L + 2) * plane(c), "save waves"));
+    bool rc = false;
+    CHK(resolve_exit_waves(c, s, &rc));
+    const int save_planes = rc ? RECOMPUTE_PLANES : c->L + 2;
+    CHK(W.save.ensure((size_t)s * save_planes * plane(c), "save waves"));
+    W.recompute = rc;
+    W.slot_bytes = plane(c) * (save_planes * sizeof(float2) + (size_t)planes(c) * sizeof(float));
     CHK(W.gslab.ensure((size_t)s * c->L * plane(c), "gradient slabs"));
     CHK(W.pslab.ensure((size_t)s * plane(c), "probe slabs"));
     CHK(W.G.ensure((size_t)c->L * plane(c), "gradient"));
@@ -438,6 +467,9 @@ int upload_impl(mset_ctx* c, const mset_params* p, const float* RVol, const floa
             const size_t src = (size_t)((y + h) % n) * n + (xx + h) % n;
             pu[(size_t)xx * n + y] = make_float2(prop2D[2 * src], prop2D[2 * src + 1]);
         }
+    double dev = 0.0;
+    for (const float2& q : pu) dev = std::max(dev, std::fabs((double)q.x * q.x + (double)q.y * q.y - 1.0));
+    c->prop_unitary = dev <= 1e-5;
     std::vector<float2> tw;
     twiddles(n, tw);
     if (RVol) HIPCHK(hipMemcpyAsync(c->V.p, RVol, sizeof(float) * n2 * c->nz, hipMemcpyHostToDevice, c->stream));
@@ -646,10 +678,13 @@ int mset_waves(mset_ctx* c, int k, float* waves_out) {
     CHK(set_method(c, MSET_METHOD_MSET));
     CHK(ensure_batch_buffers(c));
     CHK(prepare_fixed(c));
+    // every plane's wave is read back: the store layout, in slot 0
+    CHK(c->ws->save.ensure((size_t)(c->L + 2) * plane(c), "save waves"));
     PosArgs A = base_args(c);
     A.first = k;
     A.count = 1;
     A.want = 0;
+    A.recompute = 0;
     HIPCHK(launch_fill(c->ws->gslab.p, sizeof(float) * (size_t)c->L * plane(c), c->stream));
     HIPCHK(launch_batch(c->n, A, 1, c->stream));
     HIPCHK(hipMemcpyAsync(waves_out, c->ws->save.p, sizeof(float2) * (size_t)(c->L + 1) * plane(c),
@@ -665,6 +700,26 @@ int mset_run_batched(mset_ctx* c, int method, int batch, double* err_out) {
     CHK(issue_batched(c, batch));
     if (err_out) return read_errors(c, 0, c->npos, err_out);
     HIPCHK(hipStreamSynchronize(c->stream));
+    return MSET_OK;
+}
+
+int mset_set_exit_waves(mset_ctx* c, int mode) {
+    CHK(check_ctx(c));
+    if (mode != MSET_EXIT_WAVES_AUTO && mode != MSET_EXIT_WAVES_STORE && mode != MSET_EXIT_WAVES_RECOMPUTE)
+        return fail(MSET_EINVAL, "bad exit-wave mode");
+    if (mode != c->ws->exit_waves) {
+        // the next batched call sizes the wave scratch for the new variant
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->ws->save.release();
+        c->ws->exit_waves = mode;
+    }
+    return MSET_OK;
+}
+
+int mset_exit_waves(mset_ctx* c, int* resolved, size_t* bytes_per_slot) {
+    CHK(check_ctx(c));
+    if (resolved) *resolved = c->ws->recompute ? MSET_EXIT_WAVES_RECOMPUTE : MSET_EXIT_WAVES_STORE;
+    if (bytes_per_slot) *bytes_per_slot = c->ws->slot_bytes;
     return MSET_OK;
 }
 
@@ -923,6 +978,20 @@ int mset_recon_get(mset_recon* r, float* rec, float* probes, float* xlists, floa
         off += c->npos;
     }
     HIPCHK(hipStreamSynchronize(r->stream));
+    return MSET_OK;
+}
+
+int mset_recon_set_exit_waves(mset_recon* r, int mode) {
+    if (!r) return fail(MSET_EINVAL, "null reconstruction");
+    HIPCHK(hipSetDevice(r->device));
+    if (mode != MSET_EXIT_WAVES_AUTO && mode != MSET_EXIT_WAVES_STORE && mode != MSET_EXIT_WAVES_RECOMPUTE)
+        return fail(MSET_EINVAL, "bad exit-wave mode");
+    if (mode != r->ws->exit_waves) {
+        HIPCHK(hipStreamSynchronize(r->stream));
+        drop_graph(r);   // the captured pass holds the old scratch and kernel variant
+        r->ws->save.release();
+        r->ws->exit_waves = mode;
+    }
     return MSET_OK;
 }
 

@@ -195,9 +195,12 @@ struct NoOp {
 // adding up; used as a transform's post functor, the loads of its first PF elements may be issued
 // before the transform's last pass, whose work then hides their latency.  Called as an ordinary
 // functor, it runs both phases for one element.
-template <int G_, class Ld, class Ap, int PF_ = 0> struct Staged {
+template <int G_, class Ld, class Ap, int PF_ = 0, bool EAGER_ = false> struct Staged {
     static constexpr int G = G_;
     static constexpr int PF = PF_;   // elements whose loads an engine may issue ahead of the transform
+    // issue the loads of every remaining element before the first apply, so that no load is issued
+    // behind the stores an apply makes (a wait for it would also wait for those stores)
+    static constexpr bool EAGER = EAGER_;
     Ld ld;
     Ap ap;
     __device__ __forceinline__ auto load(const Pt& p) const { return ld(p); }
@@ -206,13 +209,18 @@ template <int G_, class Ld, class Ap, int PF_ = 0> struct Staged {
     }
     template <class V> __device__ __forceinline__ auto operator()(const Pt& p, V&& v) const { return ap(p, v, ld(p)); }
 };
-template <int G, int PF = 0, class Ld, class Ap> __device__ __forceinline__ Staged<G, Ld, Ap, PF> staged(Ld ld, Ap ap) {
-    return Staged<G, Ld, Ap, PF>{ld, ap};
+template <int G, int PF = 0, bool EAGER = false, class Ld, class Ap>
+__device__ __forceinline__ Staged<G, Ld, Ap, PF, EAGER> staged(Ld ld, Ap ap) {
+    return Staged<G, Ld, Ap, PF, EAGER>{ld, ap};
 }
 template <class F> struct IsStaged { static constexpr bool value = false; };
-template <int G, class Ld, class Ap, int PF> struct IsStaged<Staged<G, Ld, Ap, PF>> { static constexpr bool value = true; };
+template <int G, class Ld, class Ap, int PF, bool EG> struct IsStaged<Staged<G, Ld, Ap, PF, EG>> {
+    static constexpr bool value = true;
+};
 template <class F> struct PrefetchOf { static constexpr int value = 0; };
-template <int G, class Ld, class Ap, int PF> struct PrefetchOf<Staged<G, Ld, Ap, PF>> { static constexpr int value = PF; };
+template <int G, class Ld, class Ap, int PF, bool EG> struct PrefetchOf<Staged<G, Ld, Ap, PF, EG>> {
+    static constexpr int value = PF;
+};
 
 // ------------------------------------------------------------------ one Stockham stage
 // ROWS: transform along x (lines are rows y); otherwise along y (lines are columns x).

@@ -194,6 +194,25 @@ template <> struct RegEngine<128> {
         constexpr int G = F::G;
         static_assert(E % G == 0 && PF % G == 0 && PF <= E, "group size must divide 32 and the prefetch");
         const unsigned vb = vbase();
+        if constexpr (F::EAGER && PF < E) {
+            decltype(f.load(Pt{})) rest[E - PF];
+#pragma unroll
+            for (int i = PF; i < E; ++i) rest[i - PF] = f.load(point<COL>(i, vb));
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int g0 = 0; g0 < E; g0 += G) {
+#pragma unroll
+                for (int i = 0; i < G; ++i) {
+                    const int e = g0 + i;
+                    const Pt p = point<COL>(e, vb);
+                    const auto& o = e < PF ? ah.op[e < PF ? e : 0] : rest[e >= PF ? e - PF : 0];
+                    if constexpr (MAP) v[e] = f.apply(p, LOADG ? make_float2(0.f, 0.f) : v[e], o);
+                    else f.apply(p, v[e], o);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            return;
+        }
 #pragma unroll
         for (int g0 = 0; g0 < E; g0 += G) {
             decltype(f.load(Pt{})) op[G];

@@ -13,8 +13,9 @@ struct PosArgs {
     float sigma, dx;             // interaction constant, voxel size: phase = sigma * (dx * sum V)
     float cent;                  // round((N+1)/2), the 1-based centre of the scan coordinates
     float s1, s2, s3;            // step sizes (object, probe, position)
-    int grad_mode;
-    int sset;                    // single-slice operator: L = 1, bin = nz, no propagation, phase (sigma sum V) dx               // GRAD_REF or GRAD_EXACT
+    int grad_mode;               // GRAD_REF or GRAD_EXACT
+    int sset;                    // single-slice operator: L = 1, bin = nz, no propagation, phase (sigma sum V) dx
+    int recompute;               // K_BATCH: rebuild the waves in the adjoint instead of storing every plane
     int want;                    // WANT_PROBE | WANT_POS
     const float2* tw;            // W_N^k, k < N
     const float2* prop;          // propagator, unshifted
@@ -29,12 +30,15 @@ struct PosArgs {
     int first, count;            // K_BATCH / K_FWD: positions order[first .. first+count)
     double* err_pos;             // per position: sum over pixels of (sqrt M - sqrt I)^2
     float* inten;                // K_FWD, optional: |Psi|^2 per position, DC-centred (MATLAB layout)
-    float2* save;                // per-slot scratch: (L + 2) planes
+    float2* save;                // per-slot scratch: (L + 2) planes, or RECOMPUTE_PLANES when recomputing
     float* gslab;                // per-slot object-gradient slab, L planes (K_PLAIN: slot 0, this position)
     float2* pslab;               // K_BATCH: per-slot probe-gradient plane (Fourier domain)
     float2* planes;              // N = 256: per-slot plane buffers (2 N^2), the tiled engine's state
     float* dpos;                 // K_BATCH: per-position update terms (2 per position)
 };
+
+// per-slot scratch planes of the recompute variant: psi0, Fourier psi0, two wave planes, parked adjoint
+constexpr int RECOMPUTE_PLANES = 5;
 
 bool plane_size_supported(int n);
 int position_threads(int n);

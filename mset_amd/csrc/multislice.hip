@@ -89,14 +89,18 @@ struct AdjOp {
 };
 
 
-template <int N, int NT, int KIND, class Eng>
+template <int N, int NT, int KIND, bool RC = false, class Eng>
 __device__ __forceinline__ void process_position(const PosArgs& A, int k, int slot, Eng& E, float2* phx, float2* phy, double* red) {
     constexpr int N2 = N * N;
     const int t = threadIdx.x;
     const int L = A.L;
     const float inv_n2 = 1.0f / (float)N2;
-    float2* save = A.save + (size_t)slot * (L + 2) * N2;   // planes 0..L: waves [y][x]; L+1: Fourier psi0 [x][y]
-    float2* F0 = save + (size_t)(L + 1) * N2;
+    // Per-slot scratch.  Store: planes 0..L hold the waves psi_a [y][x], plane L+1 the Fourier psi0 [x][y].
+    // Recompute (batched kernel only): plane 0 psi0, plane 1 Fourier psi0, planes 2 and 3 the wave on
+    // either side of the current adjoint plane, plane 4 the parked adjoint wave.
+    constexpr bool recompute = RC && KIND == K_BATCH;
+    float2* save = A.save + (size_t)slot * (recompute ? RECOMPUTE_PLANES : L + 2) * N2;
+    float2* F0 = save + (size_t)(recompute ? 1 : L + 1) * N2;
     E.bind(A.planes ? A.planes + (size_t)slot * 2 * N2 : nullptr);
     const bool want_pos = (A.want & WANT_POS) != 0;
     const bool want_probe = (A.want & WANT_PROBE) != 0;
@@ -202,13 +206,15 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
     // psi_{a-1} .* t_a
     for (int a = 0; a < (multi ? L : 0); ++a) {
         E.fwd(NoOp(), staged<16, 32>([&](const Pt& p) { return pTb[p.e]; }, [&](const Pt&, float2 v, float2 o) { return cmul(v, o); }));
-        const BufC sv(save + (size_t)(a + 1) * N2, N2);
         const bool next = a + 1 < L;
+        // recompute keeps only the exit wave psi_L (plane 2), the start of the adjoint's rebuild
+        const bool keep = KIND != K_FWD && (!recompute || !next);
+        const BufC sv(save + (size_t)(recompute ? 2 : a + 1) * N2, N2);
         // the last plane has no successor: an empty view reads zeros and the product is skipped
         const BufC tn(A.trans + (size_t)(next ? a + 1 : 0) * N2, next ? N2 : 0u);
         E.inv(NoOp(), staged<16, 32>([&](const Pt& p) { return tn[p.e]; }, [&](const Pt& p, float2 v, float2 o) {
             const float2 w = cscale(v, inv_n2);
-            if (KIND != K_FWD) sv.put(p.e, w);
+            if (keep) sv.put(p.e, w);
             return next ? cmul(w, o) : w;
         }));
     }
@@ -243,10 +249,34 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
     if constexpr (KIND == K_FWD) return;
 
     // ---- adjoint sweep (STEP03_BACK.m:12-24, mset.cu:232-264)
+    // Recompute: the forward sweep is inverted plane by plane, psi_a = conj(t_a) .* ifftn(fftn(psi_{a+1})
+    // .* conj(p)), exact because the propagator and the transmission have unit modulus (the host only
+    // selects this variant for a unit-modulus propagator).  cur holds psi_{a+1} at the top of plane a.
+    float2* cur = save + 2 * (size_t)N2;
+    float2* prev = save + 3 * (size_t)N2;
+    const BufC park(save + 4 * (size_t)N2, N2);
     for (int a = L - 1; a >= 0; --a) {
-        // the single-slice operator pairs the incident wave (STEP03_BACK_SSET.m:13-14)
-        const BufC ws(save + (size_t)((exact || A.sset) ? a : a + 1) * N2, N2);
         const BufC ta(A.trans + (size_t)a * N2, N2);
+        // the single-slice operator pairs the incident wave (STEP03_BACK_SSET.m:13-14)
+        const float2* wsp = save + (size_t)((exact || A.sset) ? a : a + 1) * N2;
+        const bool rebuild = recompute && a > 0;
+        if (recompute) wsp = exact ? (a > 0 ? prev : save) : cur;
+        if constexpr (KIND == K_BATCH) {
+            if (rebuild) {
+                // park the adjoint wave (Fourier domain), rebuild psi_a into prev
+                E.each_R([&](const Pt& p, float2& v) { park.put(p.e, v); });
+                const BufC cb(cur, N2), pb(prev, N2);
+                E.template fwd<true>(staged<16>([&](const Pt& p) { return cb[p.e]; }, [&](const Pt&, float2, float2 o) { return o; }),
+                                     staged<16, 32>([&](const Pt& p) { return pTb[p.e]; },
+                                                    [&](const Pt&, float2 v, float2 o) { return cmulc(v, o); }));
+                E.inv(NoOp(), staged<16, 32>([&](const Pt& p) { return ta[p.e]; }, [&](const Pt& p, float2 v, float2 o) {
+                    const float2 w = cmulc(cscale(v, inv_n2), o);
+                    pb.put(p.e, w);
+                    return w;
+                }));
+            }
+        }
+        const BufC ws(wsp, N2);
         const BufF g(A.gslab + ((size_t)slot * L + a) * N2, N2);
         auto adjoint = [&](float2 v, float2 t, float2 w, float& gr) {
             const float2 b = cscale(v, inv_n2);
@@ -270,18 +300,27 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
                               }));
         } else {
             // accumulated into this workgroup's slab
-            E.inv(NoOp(), staged<8, 16>([&](const Pt& p) { return AdjOp{ta[p.e], ws[p.e], g[p.e]}; },
-                                    [&](const Pt& p, float2 v, const AdjOp& o) {
-                                        float gr;
-                                        const float2 r = adjoint(v, o.t, o.w, gr);
-                                        g.put(p.e, o.g + gr);
-                                        return r;
-                                    }));
+            auto post = staged<8, 16, true>([&](const Pt& p) { return AdjOp{ta[p.e], ws[p.e], g[p.e]}; },
+                                            [&](const Pt& p, float2 v, const AdjOp& o) {
+                                                float gr;
+                                                const float2 r = adjoint(v, o.t, o.w, gr);
+                                                g.put(p.e, o.g + gr);
+                                                return r;
+                                            });
+            if (rebuild)
+                E.template inv<true>(staged<16>([&](const Pt& p) { return park[p.e]; }, [&](const Pt&, float2, float2 o) { return o; }),
+                                     post);
+            else E.inv(NoOp(), post);
         }
         if (a > 0)
             E.fwd(NoOp(), staged<16, 32>([&](const Pt& p) { return pTb[p.e]; },
                                          [&](const Pt&, float2 v, float2 o) { return cmulc(v, o); }));
         else E.fwd();
+        if (rebuild) {
+            float2* t2 = cur;
+            cur = prev;
+            prev = t2;
+        }
     }
     // the engine now holds fftn(back_wave_f) (Fourier domain, R state)
 
@@ -406,13 +445,15 @@ __global__ __launch_bounds__(Cfg<N>::NT, Cfg<N>::MINB) void k_plain(PosArgs A, i
     process_position<N, NT, K_PLAIN>(A, k, 0, E, phx, phy, red);
 }
 
-template <int N>
+// RC: the recompute variant (A.recompute), a separate instantiation so that the store variant carries
+// none of its code
+template <int N, bool RC>
 __global__ __launch_bounds__(Cfg<N>::NT, Cfg<N>::MINB) void k_batch(PosArgs A) {
     constexpr int NT = Cfg<N>::NT;
     MSET_LDS(N, NT)
     for (int p = blockIdx.x; p < A.count; p += gridDim.x) {
         const int k = A.order ? A.order[A.first + p] : A.first + p;
-        process_position<N, NT, K_BATCH>(A, k, blockIdx.x, E, phx, phy, red);
+        process_position<N, NT, K_BATCH, RC>(A, k, blockIdx.x, E, phx, phy, red);
         __syncthreads();
     }
 }
@@ -501,7 +542,8 @@ hipError_t launch_plain(int n, const PosArgs& A, int k, hipStream_t st) {
 }
 
 hipError_t launch_batch(int n, const PosArgs& A, int grid, hipStream_t st) {
-    MSET_DISPATCH(n, hipLaunchKernelGGL(k_batch<NN>, dim3(grid), dim3(Cfg<NN>::NT), 0, st, A));
+    if (A.recompute) MSET_DISPATCH(n, hipLaunchKernelGGL((k_batch<NN, true>), dim3(grid), dim3(Cfg<NN>::NT), 0, st, A))
+    else MSET_DISPATCH(n, hipLaunchKernelGGL((k_batch<NN, false>), dim3(grid), dim3(Cfg<NN>::NT), 0, st, A));
     return hipGetLastError();
 }
 
@@ -518,7 +560,7 @@ hipError_t launch_plane_fft(int n, const float2* in, float2* out, const float2* 
 }
 
 hipError_t occupancy_batch(int n, int* blocks_per_cu) {
-    MSET_DISPATCH(n, return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batch<NN>, Cfg<NN>::NT, 0));
+    MSET_DISPATCH(n, return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batch<NN, false>, Cfg<NN>::NT, 0));
     return hipSuccess;
 }
 

@@ -24,8 +24,10 @@ EXPORTS = ['mset_create', 'mset_destroy', 'mset_last_error', 'mset_stream', 'mse
            'mset_synchronize', 'mset_simulate', 'mset_recon_create', 'mset_recon_destroy', 'mset_recon_set_tilt',
            'mset_recon_set_steps', 'mset_recon_iterate', 'mset_recon_error_pass', 'mset_recon_get',
            'mset_recon_snapshot', 'mset_recon_restore', 'mset_recon_profile_enable', 'mset_recon_profile_read',
-           'mset_recon_stream', 'mset_tv', 'mset_recon_tv', 'mset_device_memory', 'mset_batch_slots']
+           'mset_recon_stream', 'mset_tv', 'mset_recon_tv', 'mset_device_memory', 'mset_batch_slots',
+           'mset_set_exit_waves', 'mset_exit_waves', 'mset_recon_set_exit_waves']
 PATH_PLAIN, PATH_BATCHED = 0, 1
+EXIT_WAVES_AUTO, EXIT_WAVES_STORE, EXIT_WAVES_RECOMPUTE = 0, 1, 2
 
 
 class MsetError(RuntimeError):
@@ -76,6 +78,8 @@ def load():
         'mset_recon_stream': ([P], P),
         'mset_tv': ([P, P, P, I, I, I, F, I, P], I), 'mset_recon_tv': ([P, F, I, P], I),
         'mset_device_memory': ([I, P, P], I), 'mset_batch_slots': ([I, I, P], I),
+        'mset_set_exit_waves': ([P, I], I), 'mset_exit_waves': ([P, P, P], I),
+        'mset_recon_set_exit_waves': ([P, I], I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -258,6 +262,17 @@ class Context:
         _chk(self._L.mset_tv(self._h, _ptr(v), _ptr(out), nx, ny, nz, float(lam), int(n_iter), _ptr(e)))
         return out, float(e[0])
 
+    def set_exit_waves(self, mode):
+        """EXIT_WAVES_AUTO / _STORE / _RECOMPUTE for the batched calls (mset_set_exit_waves)."""
+        _chk(self._L.mset_set_exit_waves(self._h, int(mode)))
+
+    def exit_waves(self):
+        """(variant, per-slot scratch bytes) of the last batched call."""
+        m = C.c_int(0)
+        b = C.c_size_t(0)
+        _chk(self._L.mset_exit_waves(self._h, C.byref(m), C.byref(b)))
+        return m.value, int(b.value)
+
     # ---------------------------------------------------------------- instrumentation
     def profile(self, enable=True):
         _chk(self._L.mset_profile_enable(self._h, int(bool(enable))))
@@ -362,6 +377,10 @@ class Recon:
         e = np.zeros(1, np.float64)
         _chk(self._L.mset_recon_tv(self._h, float(lam), int(n_iter), _ptr(e)))
         return float(e[0])
+
+    def set_exit_waves(self, mode):
+        """exit-wave variant of the batched passes (EXIT_WAVES_*)"""
+        _chk(self._L.mset_recon_set_exit_waves(self._h, int(mode)))
 
     def snapshot(self):
         _chk(self._L.mset_recon_snapshot(self._h))

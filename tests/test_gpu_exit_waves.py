@@ -1,0 +1,104 @@
+"""Recomputed exit waves (mset_set_exit_waves): the batched adjoint that rebuilds psi_a by inverting the
+forward sweep gives the stored-wave gradient and the FP64 oracle's, with 5 scratch planes per resident
+position instead of L + 2.
+
+Tolerances: the forward pass is the same arithmetic in both variants (the two kernel instantiations
+may order a few roundings differently), so the losses agree to 1e-6; the rebuilt waves carry the FP32
+rounding of the inverted planes (about L ulp), so the gradients agree with the stored-wave ones to 1e-4
+relative and with the FP64 oracle to the usual 1e-3.  The adjoint recursion itself does not read the
+waves, so the probe and reference-mode position terms match the stored variant's closely too.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.cases import small_case
+from tests.conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+REBUILD_TOL = 1e-4
+GRAD_TOL = 1e-3
+
+
+def _upload(ctx, c, mode, prop=None, steps=(0.0, 0.0, 0.0)):
+    ctx.upload(c['V'], c['probe'], c['prop'] if prop is None else prop, c['xl'], c['yl'], c['M'], c['sig'],
+               c['dx'], slice_binning=c['bin'], step_size=steps, n_scan=(c['npos'], 1), grad_mode=mode)
+
+
+@pytest.mark.parametrize('n,nz,binning,mode', [(64, 16, 2, O.MODE_REF), (64, 24, 1, O.MODE_EXACT),
+                                               (32, 40, 1, O.MODE_EXACT), (128, 16, 2, O.MODE_EXACT),
+                                               (128, 32, 4, O.MODE_REF), (256, 12, 2, O.MODE_REF),
+                                               (256, 10, 2, O.MODE_EXACT)])
+def test_recomputed_waves_give_the_stored_gradient(n, nz, binning, mode):
+    from mset_amd import lib
+    c = small_case(n=n, nz=nz, binning=binning, npos=6, seed=3 * n + nz, spread=n / 8)
+    L = -(-nz // binning)
+    print('case: exit waves', n, nz, binning, mode, flush=True)
+    ctx = lib.Context(0)
+    try:
+        _upload(ctx, c, mode)
+        ctx.set_exit_waves(lib.EXIT_WAVES_STORE)
+        dV, dp, dpos, err = ctx.grad()
+        ms, bs = ctx.exit_waves()
+        ctx.set_exit_waves(lib.EXIT_WAVES_RECOMPUTE)
+        rV, rp, rpos, rerr = ctx.grad()
+        mr, br = ctx.exit_waves()
+    finally:
+        ctx.close()
+    assert (ms, mr) == (lib.EXIT_WAVES_STORE, lib.EXIT_WAVES_RECOMPUTE)
+    assert bs == n * n * ((L + 2) * 8 + L * 4) and br == n * n * (5 * 8 + L * 4)
+    assert abs(rerr / err - 1) < 1e-6                      # same forward pass
+    print('rebuilt vs stored: dV %.2e dprobe %.2e' % (rel(rV, dV), rel(rp, dp)), flush=True)
+    assert rel(rV, dV) < REBUILD_TOL
+    assert rel(rp, dp) < REBUILD_TOL
+    # position terms cancel over the plane; bound them by the largest term
+    assert np.max(np.abs(rpos - dpos)) <= REBUILD_TOL * np.max(np.abs(dpos))
+    odV, odp, odpos, _ = O.grad_fixed(c['V'], c['probe'], c['prop'], c['bprop'], c['xl'], c['yl'], c['M'],
+                                      c['bin'], c['sig'], c['dx'], mode=mode)
+    assert rel(rV, odV) < GRAD_TOL
+    assert rel(rp, odp) < GRAD_TOL
+    assert rel(rpos, odpos) < GRAD_TOL
+
+
+def test_recompute_needs_a_unit_modulus_propagator_and_auto_stores():
+    from mset_amd import lib
+    c = small_case(n=32, nz=12, binning=1, npos=3, seed=5, spread=4.0)
+    ctx = lib.Context(0)
+    try:
+        _upload(ctx, c, O.MODE_REF, prop=0.9 * c['prop'])
+        ctx.set_exit_waves(lib.EXIT_WAVES_RECOMPUTE)
+        with pytest.raises(lib.MsetError) as e:
+            ctx.grad()
+        assert e.value.code == lib.MSET_EINVAL
+        ctx.set_exit_waves(lib.EXIT_WAVES_AUTO)
+        ctx.grad()
+        assert ctx.exit_waves()[0] == lib.EXIT_WAVES_STORE
+        # a unit-modulus propagator at a small size: the store scratch is far below a quarter of the device
+        _upload(ctx, c, O.MODE_REF)
+        ctx.grad()
+        assert ctx.exit_waves()[0] == lib.EXIT_WAVES_STORE
+    finally:
+        ctx.close()
+
+
+def test_recon_pass_with_recomputed_waves():
+    """A graph-captured batched update pass of a two-tilt series: rebuilt waves give the stored-wave
+    volume, and switching the variant drops the captured graph."""
+    from mset_amd import lib
+    c = small_case(n=64, nz=16, binning=2, npos=8, seed=11, spread=6.0)
+    out = {}
+    for mode in (lib.EXIT_WAVES_STORE, lib.EXIT_WAVES_RECOMPUTE):
+        r = lib.Recon(c['V'], c['prop'], 2, c['sig'], c['dx'], slice_binning=c['bin'], step_size=(50.0, 0.0, 0.0))
+        try:
+            r.set_exit_waves(mode)
+            for j, ang in enumerate(((0.0, -20.0, 0.0), (0.0, 25.0, 0.0))):
+                r.set_tilt(j, ang, c['probe'], c['xl'], c['yl'], c['M'], n_scan=(c['npos'], 1))
+            for _ in range(2):
+                r.iterate(lib.PATH_BATCHED, 4, use_graph=True, want_error=False)
+            out[mode] = r.get()[0]
+        finally:
+            r.close()
+    d = out[lib.EXIT_WAVES_RECOMPUTE] - c['V']
+    ds = out[lib.EXIT_WAVES_STORE] - c['V']
+    assert np.linalg.norm(d - ds) <= REBUILD_TOL * np.linalg.norm(ds) + 1e-6 * np.linalg.norm(c['V'])
