@@ -1,6 +1,6 @@
 """Benchmark of the multislice forward + adjoint path on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--batch B] [--gpus 1]
+    python bench.py [--steps K] [--warmup W] [--workload C2|C3] [--batch B] [--no-autotune] [--gpus 1]
 
 Workload: the GPU configuration of the reference's plain method (64^3 volume at 0.4 A in a 128 x 128
 field, probe 128 x 128 at 300 kV / 21 mrad, 32 x 32 scan with sub-pixel jitter, 11 tilts from -50 to
@@ -9,7 +9,8 @@ One step = one full update pass of the reconstruction over all 11 tilts: for eac
 Fourier-shear rotation of the volume, forward + adjoint for all 1024 positions with the object update,
 the forward rotation and positivity (Run_MSET_recon.m:119-187).  Inputs are resident in HBM before
 timing starts.  The headline path runs positions in mini-batches of B per launch (one workgroup per
-position, 256 in flight) replayed from a captured hipGraph; the plain path (one position per launch,
+position, up to one per CU in flight) replayed from a captured hipGraph; B and the persistent grid are
+picked by the autotuner (mset_amd/autotune.py) before timing.  The plain path (one position per launch,
 sequential updates, the reference's semantics) is measured on the same workload and reported beside it.
 
 Prints one JSON line.
@@ -65,7 +66,11 @@ def main():
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--workload', default='C2', choices=['C2', 'C3'],
                     help='C2 (default): the configuration the metric is quoted on; C3: 256^2 probe, uneven scans')
-    ap.add_argument('--batch', type=int, default=None, help='positions per mini-batch (default 1024 C2, 512 C3)')
+    ap.add_argument('--batch', type=int, default=None,
+                    help='positions per mini-batch (default: picked by the autotuner from 256..1024 for C2, '
+                         '64..512 for C3)')
+    ap.add_argument('--no-autotune', action='store_true',
+                    help='skip the autotuner: batch 1024 (C2) / 512 (C3) on every resident workgroup')
     ap.add_argument('--no-graph', action='store_true')
     ap.add_argument('--plain-tilts', type=int, default=None, help='tilts of the plain path to time (0 = skip; '
                     'default 1 for C2, 0 for C3)')
@@ -81,8 +86,11 @@ def main():
 
     from mset_amd import lib, synth
     big = args.workload == 'C3'
-    if args.batch is None:
-        args.batch = 512 if big else 1024
+    batches = (64, 128, 256, 512) if big else (256, 512, 1024)
+    if args.batch is not None:
+        batches = (args.batch,)
+    elif args.no_autotune:
+        batches = (max(batches),)
     if args.plain_tilts is None:
         args.plain_tilts = 0 if big else 1
     if args.cpu_positions is None:
@@ -108,8 +116,17 @@ def main():
 
     units = wl.pos_slices()
     use_graph = not args.no_graph
-    log('batched path: batch %d, graph %s' % (args.batch, use_graph))
     r = make_recon()
+    tuned = None
+    if not args.no_autotune:
+        from mset_amd import autotune
+        tuned = autotune.tune_recon(r, batches=batches, log=log)
+        args.batch = tuned['batch']
+        tuned = dict(batch=tuned['batch'], slots=tuned['slots'],
+                     pass_ms={'%d/%d' % (sl, b): round(dt, 2) for sl, b, dt, _ in tuned['table']})
+    else:
+        args.batch = batches[0]
+    log('batched path: batch %d, graph %s' % (args.batch, use_graph))
     for _ in range(args.warmup):
         r.iterate(lib.PATH_BATCHED, args.batch, use_graph=use_graph, want_error=False)
     t0 = time.perf_counter()
@@ -164,7 +181,7 @@ def main():
         data='synthetic: seeded FCC Gaussian-atom potential, device-simulated Poisson patterns',
         config=dict(workload=WORKLOADS[args.workload],
                     path='batched', batch=args.batch, hip_graph=use_graph, positions_per_step=wl.positions(),
-                    planes=wl.L, pos_slices_per_step=units, step_size=list(steps),
+                    planes=wl.L, pos_slices_per_step=units, step_size=list(steps), autotune=tuned,
                     error_after=err_after, rotate_ms_per_step=ms_r),
         roofline=dict(bound='hbm', achieved=achieved, peak=HBM_PEAK_GBS, unit='GB/s', frac=achieved / HBM_PEAK_GBS,
                       traffic=traffic, kernel='position_batch', kernel_avg_ms=avg_ms,

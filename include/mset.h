@@ -120,6 +120,10 @@ int mset_waves(mset_ctx* ctx, int k, float* waves_out);
 #define MSET_EXIT_WAVES_RECOMPUTE 2
 int mset_set_exit_waves(mset_ctx* ctx, int mode);
 int mset_exit_waves(mset_ctx* ctx, int* resolved, size_t* bytes_per_slot);
+/* Cap on the persistent grid of the batched kernels (resident positions, each with its own scratch);
+ * 0 (the default) = every workgroup the device keeps resident (mset_batch_slots).  A tuning knob with
+ * no counterpart in the reference: fewer slots hold less scratch and may keep more of it in cache. */
+int mset_set_slot_limit(mset_ctx* ctx, int slots);
 
 /* Mini-batch variant of 'run_mset': positions are taken in order in batches of `batch`; each batch's
  * gradient is formed at the state left by the previous batch and then applied.  batch == 1 equals
@@ -185,6 +189,8 @@ int mset_recon_get(mset_recon* r, float* rec, float* probes, float* xlists, floa
 int mset_recon_tv(mset_recon* r, float lambda, int n_iter, double* tv_error);
 /* exit-wave variant of the tilt series' batched passes (MSET_EXIT_WAVES_*, as mset_set_exit_waves) */
 int mset_recon_set_exit_waves(mset_recon* r, int mode);
+/* slot cap of the tilt series' batched passes (as mset_set_slot_limit) */
+int mset_recon_set_slot_limit(mset_recon* r, int slots);
 /* BLS rollback (Run_MSET_recon.m:113-115, 245-247) */
 int mset_recon_snapshot(mset_recon* r);
 int mset_recon_restore(mset_recon* r);

@@ -1,0 +1,117 @@
+"""Autotuner of the batched path: picks the persistent grid and the mini-batch size by timing them on the
+workload itself.
+
+Tunables (both leave the kernels' arithmetic unchanged):
+  slots  the persistent grid of the batched kernels (mset_set_slot_limit).  Fewer resident positions
+         hold less scratch (saved waves, gradient slabs and, at 256 x 256, the plane buffers) and keep
+         more of it in L2 / MALL; more of them hide memory latency better.
+  batch  positions per launch of an update pass (the mini-batch size), from the range the caller
+         allows: it sets how many launches, reductions and updates a pass makes and how well the last
+         wave of workgroups fills the device.
+The plane engines themselves (radix, LDS tile, workgroup shape) are fixed per plane size at compile
+time; the grid is what remains to tune at run time.
+
+Each candidate runs one update pass from a snapshot of the tilt series, which is restored afterwards,
+so tuning leaves the reconstruction unchanged.  The kernels are timed with HIP events on the series'
+stream; the first pass at each grid size (which sizes the scratch) is not timed.  Results are cached per
+shape in memory and, if a path is given, in a JSON file.
+"""
+import json
+import os
+import time
+
+from . import lib
+
+_CACHE = {}
+
+
+def slot_candidates(n, fracs=(1.0, 0.75, 0.5), device=0):
+    """Grid sizes to try: fractions of the resident workgroups, rounded to whole CUs' worth."""
+    full = lib.batch_slots(n, device)
+    out = []
+    for f in fracs:
+        s = max(1, int(round(full * f)))
+        if s not in out:
+            out.append(s)
+    return out
+
+
+def _key(recon, batches, slots):
+    return json.dumps(dict(shape=list(recon.shape), npos=list(recon.npos), batches=list(batches),
+                           slots=list(slots)), sort_keys=True)
+
+
+def tune_recon(recon, batches=(64, 128, 256, 512), slots=None, cache_path=None, log=None):
+    """Time one batched update pass per (slots, batch) candidate on `recon` and leave the fastest grid
+    set on it.  Returns dict(batch, slots, ms, table=[(slots, batch, pass ms, position-kernel ms)])."""
+    if slots is None:
+        slots = slot_candidates(recon.shape[1])
+    key = _key(recon, batches, slots)
+    if key in _CACHE:
+        best = _CACHE[key]
+        recon.set_slot_limit(best['slots'])
+        return best
+    if cache_path and os.path.exists(cache_path):
+        with open(cache_path) as fh:
+            disk = json.load(fh)
+        if key in disk:
+            _CACHE[key] = disk[key]
+            recon.set_slot_limit(disk[key]['slots'])
+            return disk[key]
+    recon.snapshot()
+    table = []
+    try:
+        for s in slots:
+            recon.set_slot_limit(s)
+            recon.iterate(lib.PATH_BATCHED, max(batches), use_graph=False, want_error=False)   # sizes scratch
+            recon.restore()
+            for b in batches:
+                recon.profile(True)
+                t0 = time.perf_counter()
+                recon.iterate(lib.PATH_BATCHED, b, use_graph=False, want_error=False)
+                dt = (time.perf_counter() - t0) * 1e3
+                _, kms = recon.profile_read('position_batch')
+                recon.profile_read('rotate')                # consumed here, not left for the caller
+                recon.profile(False)
+                recon.restore()
+                table.append((s, b, dt, kms))
+                if log:
+                    log('autotune: slots %d batch %d: pass %.2f ms, position kernels %.2f ms' % (s, b, dt, kms))
+    finally:
+        recon.profile(False)
+        recon.restore()
+    s, b, dt, kms = min(table, key=lambda r: r[2])
+    best = dict(batch=b, slots=s, ms=dt, table=table)
+    recon.set_slot_limit(s)
+    _CACHE[key] = best
+    if cache_path:
+        disk = {}
+        if os.path.exists(cache_path):
+            with open(cache_path) as fh:
+                disk = json.load(fh)
+        disk[key] = best
+        with open(cache_path, 'w') as fh:
+            json.dump(disk, fh)
+    return best
+
+
+def tune_grad(ctx, n, slots=None, first=0, count=None, repeats=2, log=None):
+    """Grid size of the fixed-state gradient (mset_grad) on `ctx`; leaves the fastest set.  Returns
+    dict(slots, ms, table=[(slots, kernel ms)])."""
+    if slots is None:
+        slots = slot_candidates(n)
+    table = []
+    for s in slots:
+        ctx.set_slot_limit(s)
+        ctx.grad(first, count)                              # sizes the scratch
+        ctx.profile(True)
+        for _ in range(repeats):
+            ctx.grad(first, count)
+        k, ms = ctx.profile_read('position_batch')
+        ctx.profile(False)
+        table.append((s, ms / max(k, 1)))
+        if log:
+            log('autotune: slots %d: %.3f ms per launch' % (s, ms / max(k, 1)))
+    s, ms = min(table, key=lambda r: r[1])
+    ctx.set_slot_limit(s)
+    return dict(slots=s, ms=ms, table=table)

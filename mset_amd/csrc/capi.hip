@@ -85,6 +85,7 @@ struct Workspace {
     DevBuf<float2> save, trans, probeF, pslab, PG, planes;
     DevBuf<float> gslab, G;
     int exit_waves = MSET_EXIT_WAVES_AUTO;   // requested exit-wave variant of the batched kernels
+    int slot_limit = 0;                      // cap on the persistent grid of the batched kernels (0: none)
     bool recompute = false;                  // the variant of the last batched call
     size_t slot_bytes = 0;                   // its per-slot scratch (waves + gradient slab)
     RotWork rot;
@@ -241,6 +242,7 @@ int batch_slots(mset_ctx* c) {
     int per_cu = 1;
     if (occupancy_batch(c->n, &per_cu) != hipSuccess || per_cu < 1) per_cu = 1;
     int s = c->cus * per_cu;
+    if (c->ws->slot_limit > 0 && s > c->ws->slot_limit) s = c->ws->slot_limit;
     if (s > c->npos) s = c->npos;
     return s < 1 ? 1 : s;
 }
@@ -716,6 +718,13 @@ int mset_set_exit_waves(mset_ctx* c, int mode) {
     return MSET_OK;
 }
 
+int mset_set_slot_limit(mset_ctx* c, int slots) {
+    CHK(check_ctx(c));
+    if (slots < 0) return fail(MSET_EINVAL, "slot limit must be >= 0");
+    c->ws->slot_limit = slots;
+    return MSET_OK;
+}
+
 int mset_exit_waves(mset_ctx* c, int* resolved, size_t* bytes_per_slot) {
     CHK(check_ctx(c));
     if (resolved) *resolved = c->ws->recompute ? MSET_EXIT_WAVES_RECOMPUTE : MSET_EXIT_WAVES_STORE;
@@ -991,6 +1000,18 @@ int mset_recon_set_exit_waves(mset_recon* r, int mode) {
         drop_graph(r);   // the captured pass holds the old scratch and kernel variant
         r->ws->save.release();
         r->ws->exit_waves = mode;
+    }
+    return MSET_OK;
+}
+
+int mset_recon_set_slot_limit(mset_recon* r, int slots) {
+    if (!r) return fail(MSET_EINVAL, "null reconstruction");
+    if (slots < 0) return fail(MSET_EINVAL, "slot limit must be >= 0");
+    HIPCHK(hipSetDevice(r->device));
+    if (slots != r->ws->slot_limit) {
+        HIPCHK(hipStreamSynchronize(r->stream));
+        drop_graph(r);   // the captured pass holds the old grid
+        r->ws->slot_limit = slots;
     }
     return MSET_OK;
 }

@@ -25,7 +25,8 @@ EXPORTS = ['mset_create', 'mset_destroy', 'mset_last_error', 'mset_stream', 'mse
            'mset_recon_set_steps', 'mset_recon_iterate', 'mset_recon_error_pass', 'mset_recon_get',
            'mset_recon_snapshot', 'mset_recon_restore', 'mset_recon_profile_enable', 'mset_recon_profile_read',
            'mset_recon_stream', 'mset_tv', 'mset_recon_tv', 'mset_device_memory', 'mset_batch_slots',
-           'mset_set_exit_waves', 'mset_exit_waves', 'mset_recon_set_exit_waves']
+           'mset_set_exit_waves', 'mset_exit_waves', 'mset_recon_set_exit_waves', 'mset_set_slot_limit',
+           'mset_recon_set_slot_limit']
 PATH_PLAIN, PATH_BATCHED = 0, 1
 EXIT_WAVES_AUTO, EXIT_WAVES_STORE, EXIT_WAVES_RECOMPUTE = 0, 1, 2
 
@@ -79,7 +80,8 @@ def load():
         'mset_tv': ([P, P, P, I, I, I, F, I, P], I), 'mset_recon_tv': ([P, F, I, P], I),
         'mset_device_memory': ([I, P, P], I), 'mset_batch_slots': ([I, I, P], I),
         'mset_set_exit_waves': ([P, I], I), 'mset_exit_waves': ([P, P, P], I),
-        'mset_recon_set_exit_waves': ([P, I], I),
+        'mset_recon_set_exit_waves': ([P, I], I), 'mset_set_slot_limit': ([P, I], I),
+        'mset_recon_set_slot_limit': ([P, I], I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -266,6 +268,10 @@ class Context:
         """EXIT_WAVES_AUTO / _STORE / _RECOMPUTE for the batched calls (mset_set_exit_waves)."""
         _chk(self._L.mset_set_exit_waves(self._h, int(mode)))
 
+    def set_slot_limit(self, slots):
+        """cap on the batched kernels' persistent grid (0 = all resident workgroups)"""
+        _chk(self._L.mset_set_slot_limit(self._h, int(slots)))
+
     def exit_waves(self):
         """(variant, per-slot scratch bytes) of the last batched call."""
         m = C.c_int(0)
@@ -381,6 +387,10 @@ class Recon:
     def set_exit_waves(self, mode):
         """exit-wave variant of the batched passes (EXIT_WAVES_*)"""
         _chk(self._L.mset_recon_set_exit_waves(self._h, int(mode)))
+
+    def set_slot_limit(self, slots):
+        """cap on the batched passes' persistent grid (0 = all resident workgroups)"""
+        _chk(self._L.mset_recon_set_slot_limit(self._h, int(slots)))
 
     def snapshot(self):
         _chk(self._L.mset_recon_snapshot(self._h))

@@ -6,7 +6,9 @@ one job (an update pass over all its tilts, followed by its loss).  Requests are
 
   * admission by device memory: a job becomes resident when its estimated footprint fits the budget;
     when it does not, idle resident jobs are evicted (least recently used first) -- their state is
-    copied to the host and restored exactly on re-admission;
+    copied to the host and restored exactly on re-admission.  The earliest request that waits for
+    memory holds back the resident jobs it needs evicted (no new requests start on them), so a large
+    job is not starved by a stream of requests for small resident ones;
   * up to `workers` requests run at once, each job on its own HIP stream, so work of different jobs
     overlaps on the device; one job never runs two requests at a time, so its iterations stay ordered;
   * every request's latency (arrival to completion) is recorded; p50 / p99 and the aggregate
@@ -204,10 +206,13 @@ class Server:
     # ------------------------------------------------------------------ scheduling (cv held)
     def _pick(self):
         """The earliest pending request whose job is idle and resident, or admissible after evicting idle
-        jobs; returns (request, jobs to evict) or (None, None)."""
+        jobs; returns (request, jobs to evict) or (None, None).  The first request that cannot be
+        admitted yet reserves the resident jobs (least recently used first) whose eviction would make
+        room for it: later requests of those jobs wait."""
+        held = set()
         for i, req in enumerate(self.pending):
             job = self.jobs[req.job_id]
-            if job.busy:
+            if job.busy or req.job_id in held:
                 continue
             if job.recon is not None:
                 return self.pending.pop(i), []
@@ -223,6 +228,13 @@ class Server:
                     need -= j.bytes
             if need <= 0:
                 return self.pending.pop(i), evict
+            if not held:
+                need = job.bytes - (self.budget - self.resident_bytes)
+                for j in sorted((j for j in self.jobs.values() if j.recon is not None), key=lambda j: j.last_used):
+                    if need <= 0:
+                        break
+                    held.add(j.spec.job_id)
+                    need -= j.bytes
         return None, None
 
     def _worker(self):

@@ -97,3 +97,30 @@ def test_draw_jobs_respects_the_configuration():
         assert 8 <= len(s.tilts) <= 30 and 16 <= s.scan <= 40
     t, jobs = serve.poisson_arrivals(200, 20.0, 16, 1)
     assert abs(200 / t[-1] - 20.0) < 5.0 and np.all(np.diff(t) >= 0) and jobs.max() < 16
+
+
+def test_waiting_large_job_is_not_starved_by_resident_small_ones():
+    """Two small jobs fit together; the large one needs both evicted.  Requests of the small jobs keep
+    arriving behind the large job's request: once it waits, the small jobs take no new requests, so the
+    large job runs before the later small requests, which then need re-admission."""
+    FakeRecon.live = FakeRecon.peak = 0
+    specs = [serve.JobSpec(job_id=0, vol=32, P=64, tilts=[0.0], scan=8, seed=0),
+             serve.JobSpec(job_id=1, vol=32, P=64, tilts=[0.0], scan=8, seed=1),
+             serve.JobSpec(job_id=2, vol=64, P=128, tilts=[0.0, 10.0, 20.0], scan=8, seed=2)]
+    small = serve.footprint_bytes(specs[0], 8)
+    big = serve.footprint_bytes(specs[2], 8)
+    assert big > 2 * small
+    srv = serve.Server(specs, workers=2, budget_bytes=big + small // 2, recon_factory=FakeRecon,
+                       slots_of=lambda n: 8)
+    order = [0, 1, 2] + [0, 1] * 6
+    reqs = [srv.submit(j) for j in order]
+    for r in reqs:
+        assert r.done.wait(10)
+    srv.close()
+    assert all(not r.failure for r in reqs)
+    big_req = reqs[2]
+    # every small request submitted after the large one started after the large one's admission
+    for r in reqs[3:]:
+        assert r.start >= big_req.start
+    rep = srv.report()
+    assert rep['evictions'] >= 2 and rep['admissions'] >= 5
