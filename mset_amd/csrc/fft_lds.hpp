@@ -316,6 +316,7 @@ template <int N, int NT_> struct LdsEngine {
     static constexpr int NT = NT_;
     static constexpr int S = Pitch<N>::S;
     static constexpr int LDS_ELEMS = N * S;
+    static constexpr bool FUSED = false;   // no fused passes: the plane is on chip
     float2* s;
     const float2* tw;
 

@@ -66,6 +66,7 @@ template <int N> struct RegEngine;
 template <> struct RegEngine<128> {
     static constexpr int N = 128, E = 32, NT = 512, SP = 129;
     static constexpr int LDS_ELEMS = N * SP;
+    static constexpr bool FUSED = false;   // no fused passes: the plane is in registers
     float2* s;
     const float2* tw;  // W_128^k table in LDS
     float2 v[E];

@@ -173,6 +173,10 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
     // the forward sweep ends with the next plane's transmission, whose operands are loaded ahead of the
     // last pass: the loads are then issued before the wave's stores and never wait behind them.
     const bool multi = !A.sset && L > 0;
+    // An engine whose plane lives in global memory fuses the passes that meet between transforms
+    // (row_step, col_step): from here to the end of the adjoint the plane alternates between the
+    // half-forward and half-inverse states.  The recompute variant keeps the plain transforms.
+    constexpr bool FUSE = Eng::FUSED && !recompute;
     auto psi0_post = staged<16, 32>([&](const Pt& p) { return tr0[p.e]; }, [&](const Pt& p, float2 v, float2 o) {
         float2 w = cscale(v, inv_n2);
         if (KIND != K_FWD) sv0.put(p.e, w);
@@ -190,33 +194,46 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
                                  F0b.put(p.e, f);
                                  return f;
                              });
-        E.inv(NoOp(), psi0_post);
+        if constexpr (FUSE) {
+            E.row_first(NoOp());
+            E.col_step(psi0_post);
+        } else {
+            E.inv(NoOp(), psi0_post);
+        }
     } else {
         const BufC pf(A.probeF, N2);                        // fftn(probe), [x][y]
-        E.template inv<true>(staged<16>([&](const Pt& p) { return pf[p.e]; },
-                                        [&](const Pt& p, float2, float2 o) {
-                                            const float2 f = cmul(o, phase(p.y, p.x));
-                                            if (KIND == K_BATCH) F0b.put(p.e, f);
-                                            return f;
-                                        }),
-                             psi0_post);
+        auto spectrum = staged<16>([&](const Pt& p) { return pf[p.e]; }, [&](const Pt& p, float2, float2 o) {
+            const float2 f = cmul(o, phase(p.y, p.x));
+            if (KIND == K_BATCH) F0b.put(p.e, f);
+            return f;
+        });
+        if constexpr (FUSE) {
+            E.template row_first<true>(spectrum);
+            E.col_step(psi0_post);
+        } else {
+            E.template inv<true>(spectrum, psi0_post);
+        }
     }
 
     // ---- forward sweep: psi_a = ifftn(fftn(psi_{a-1} .* t_a) .* p); the engine enters plane a holding
     // psi_{a-1} .* t_a
     for (int a = 0; a < (multi ? L : 0); ++a) {
-        E.fwd(NoOp(), staged<16, 32>([&](const Pt& p) { return pTb[p.e]; }, [&](const Pt&, float2 v, float2 o) { return cmul(v, o); }));
+        auto byp = staged<16, 32>([&](const Pt& p) { return pTb[p.e]; }, [&](const Pt&, float2 v, float2 o) { return cmul(v, o); });
+        if constexpr (FUSE) E.row_step(byp);
+        else E.fwd(NoOp(), byp);
         const bool next = a + 1 < L;
         // recompute keeps only the exit wave psi_L (plane 2), the start of the adjoint's rebuild
         const bool keep = KIND != K_FWD && (!recompute || !next);
         const BufC sv(save + (size_t)(recompute ? 2 : a + 1) * N2, N2);
         // the last plane has no successor: an empty view reads zeros and the product is skipped
         const BufC tn(A.trans + (size_t)(next ? a + 1 : 0) * N2, next ? N2 : 0u);
-        E.inv(NoOp(), staged<16, 32>([&](const Pt& p) { return tn[p.e]; }, [&](const Pt& p, float2 v, float2 o) {
+        auto post = staged<16, 32>([&](const Pt& p) { return tn[p.e]; }, [&](const Pt& p, float2 v, float2 o) {
             const float2 w = cscale(v, inv_n2);
             if (keep) sv.put(p.e, w);
             return next ? cmul(w, o) : w;
-        }));
+        });
+        if constexpr (FUSE) E.col_step(post);
+        else E.inv(NoOp(), post);
     }
 
     // ---- exit wave, intensity, amplitude residual (STEP02_FORWARD_4D.m:36-49), r .* conj(p) fused
@@ -224,7 +241,7 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
     {
         const BufF MT(A.sqrtM + (size_t)k * N2, N2);        // sqrt(M), [x][y]
         float* Iout = A.inten ? A.inten + (size_t)k * N2 : nullptr;
-        E.fwd(NoOp(), staged<16, 16>(
+        auto exitf = staged<16, 16>(
                           [&](const Pt& p) {
                               const float m = MT[p.e];
                               if (KIND == K_FWD) return make_float3(m, 0.f, 0.f);
@@ -242,7 +259,14 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
                               if (KIND == K_FWD) return v;
                               const float f = 1.0f - m * __builtin_amdgcn_rsqf(I + 1e-30f);
                               return cmulc(make_float2(v.x * f, v.y * f), make_float2(o.y, o.z));
-                          }));
+                          });
+        if constexpr (FUSE) {
+            // the adjoint's first row pass rides along (none for the error pass)
+            if constexpr (KIND == K_FWD) E.row_last(exitf);
+            else E.row_step(exitf);
+        } else {
+            E.fwd(NoOp(), exitf);
+        }
     }
     err = block_sum<NT>(err, red);
     if (t == 0) A.err_pos[k] = err;
@@ -287,17 +311,19 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
         };
         if constexpr (KIND == K_PLAIN) {
             // the gradient plane of this position (applied to the bin after the sweep)
-            E.inv(NoOp(), staged<16, 16>(
-                              [&](const Pt& p) {
-                                  const float2 tt = ta[p.e], ww = ws[p.e];
-                                  return make_float4(tt.x, tt.y, ww.x, ww.y);
-                              },
-                              [&](const Pt& p, float2 v, float4 o) {
-                                  float gr;
-                                  const float2 r = adjoint(v, make_float2(o.x, o.y), make_float2(o.z, o.w), gr);
-                                  g.put(p.e, gr);
-                                  return r;
-                              }));
+            auto post = staged<16, 16>(
+                [&](const Pt& p) {
+                    const float2 tt = ta[p.e], ww = ws[p.e];
+                    return make_float4(tt.x, tt.y, ww.x, ww.y);
+                },
+                [&](const Pt& p, float2 v, float4 o) {
+                    float gr;
+                    const float2 r = adjoint(v, make_float2(o.x, o.y), make_float2(o.z, o.w), gr);
+                    g.put(p.e, gr);
+                    return r;
+                });
+            if constexpr (FUSE) E.col_step(post);
+            else E.inv(NoOp(), post);
         } else {
             // accumulated into this workgroup's slab
             auto post = staged<8, 16, true>([&](const Pt& p) { return AdjOp{ta[p.e], ws[p.e], g[p.e]}; },
@@ -307,15 +333,21 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
                                                 g.put(p.e, o.g + gr);
                                                 return r;
                                             });
-            if (rebuild)
+            if constexpr (FUSE) E.col_step(post);
+            else if (rebuild)
                 E.template inv<true>(staged<16>([&](const Pt& p) { return park[p.e]; }, [&](const Pt&, float2, float2 o) { return o; }),
                                      post);
             else E.inv(NoOp(), post);
         }
-        if (a > 0)
-            E.fwd(NoOp(), staged<16, 32>([&](const Pt& p) { return pTb[p.e]; },
-                                         [&](const Pt&, float2 v, float2 o) { return cmulc(v, o); }));
-        else E.fwd();
+        if (a > 0) {
+            auto bypc = staged<16, 32>([&](const Pt& p) { return pTb[p.e]; },
+                                       [&](const Pt&, float2 v, float2 o) { return cmulc(v, o); });
+            if constexpr (FUSE) E.row_step(bypc);
+            else E.fwd(NoOp(), bypc);
+        } else {
+            if constexpr (FUSE) E.row_last(NoOp());
+            else E.fwd();
+        }
         if (rebuild) {
             float2* t2 = cur;
             cur = prev;
