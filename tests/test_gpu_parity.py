@@ -247,3 +247,30 @@ def test_sset_batched_step_is_the_fixed_state_gradient(mset):
                                   c['sig'], c['dx'], mode=O.MODE_REF | O.MODE_SSET)
     assert abs(e / (oep.sum() / c['npos']) - 1) < LOSS_TOL
     assert_update_close(V, c['V'] + steps[0] * odV[0][None], c['V'], 1)
+
+
+@pytest.mark.parametrize('n,nz,binning,npos,mode', [(64, 3, 5, 1, O.MODE_REF), (32, 1, 1, 2, O.MODE_EXACT),
+                                                    (128, 2, 1, 1, O.MODE_REF), (256, 3, 4, 1, O.MODE_EXACT)])
+def test_single_plane_and_single_position_edges(mset, n, nz, binning, npos, mode):
+    """One plane (binning >= nz, clamped as STEP00_INIT.m:127-129) and one position: the batched grid
+    shrinks to the positions and the sweeps have a single step."""
+    c = small_case(n=n, nz=nz, binning=binning, npos=npos, seed=7 * n + nz, spread=n / 8)
+    print('case: edge', n, nz, binning, npos, mode, flush=True)
+    _upload_case(mset, c, mode=mode)
+    dV, dp, dpos, err = mset.grad()
+    odV, odp, odpos, oep = O.grad_fixed(c['V'], c['probe'], c['prop'], c['bprop'], c['xl'], c['yl'], c['M'],
+                                        c['bin'], c['sig'], c['dx'], mode=mode)
+    assert dV.shape[0] == -(-nz // min(binning, nz))
+    assert abs(err / (oep.sum() / npos) - 1) < LOSS_TOL
+    assert rel(dV, odV) < GRAD_TOL
+    assert rel(dp, odp) < GRAD_TOL
+    assert rel(dpos, odpos) < GRAD_TOL
+    # the sequential path agrees with its oracle on the same state
+    steps = (30.0, 2e-3, 2e-2)
+    _upload_case(mset, c, steps=steps, mode=mode)
+    V2, p2, x2, y2, e2 = mset.run()
+    oV, op, ox, oy, oe, _ = O.run_tilt(c['V'], c['probe'], c['prop'], c['bprop'], c['xl'], c['yl'], c['M'], npos, 1,
+                                       c['bin'], c['sig'], c['dx'], steps, mode=mode)
+    assert abs(e2 / oe - 1) < LOSS_TOL
+    assert_update_close(V2, oV, c['V'], npos)
+    assert_update_close(p2, op, c['probe'], npos)
