@@ -353,8 +353,7 @@ int issue_grad(mset_ctx* c, int first, int count, bool apply, int want) {
     A.first = first;
     A.count = count;
     A.want = want;
-    HIPCHK(launch_fill(W.gslab.p, sizeof(float) * (size_t)grid * planes(c) * plane(c), c->stream));
-    if (want & WANT_PROBE) HIPCHK(launch_fill(W.pslab.p, sizeof(float2) * (size_t)grid * plane(c), c->stream));
+    // every slot of the grid processes at least one position, and its first one writes the slot's slabs
     {
         Timed t(c, "position_batch");
         HIPCHK(launch_batch(c->n, A, grid, c->stream));
@@ -687,7 +686,6 @@ int mset_waves(mset_ctx* c, int k, float* waves_out) {
     A.count = 1;
     A.want = 0;
     A.recompute = 0;
-    HIPCHK(launch_fill(c->ws->gslab.p, sizeof(float) * (size_t)c->L * plane(c), c->stream));
     HIPCHK(launch_batch(c->n, A, 1, c->stream));
     HIPCHK(hipMemcpyAsync(waves_out, c->ws->save.p, sizeof(float2) * (size_t)(c->L + 1) * plane(c),
                           hipMemcpyDeviceToHost, c->stream));

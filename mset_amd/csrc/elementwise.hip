@@ -53,10 +53,23 @@ __global__ void k_fill_bytes(unsigned char* p, size_t n) {
 }
 
 // out[i] = sum_s slab[s][i], s ascending (fixed order: bitwise repeatable)
+// Slabs are summed in slot order (s = 0, 1, ...), so the result is the same for every launch shape;
+// U loads are in flight at a time.
 __global__ void k_reduce(const float4* __restrict__ slab, int nslab, size_t n4, float4* __restrict__ out) {
+    constexpr int U = 8;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x) {
         float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
-        for (int s = 0; s < nslab; ++s) {
+        int s = 0;
+        for (; s + U <= nslab; s += U) {
+            float4 v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = slab[(size_t)(s + u) * n4 + i];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                a.x += v[u].x; a.y += v[u].y; a.z += v[u].z; a.w += v[u].w;
+            }
+        }
+        for (; s < nslab; ++s) {
             const float4 v = slab[(size_t)s * n4 + i];
             a.x += v.x; a.y += v.y; a.z += v.z; a.w += v.w;
         }

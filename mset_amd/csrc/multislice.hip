@@ -89,8 +89,11 @@ struct AdjOp {
 };
 
 
+// first: the slot's first position of the launch, which writes the slot's slabs instead of adding to
+// them (so the slabs need no clearing launch)
 template <int N, int NT, int KIND, bool RC = false, class Eng>
-__device__ __forceinline__ void process_position(const PosArgs& A, int k, int slot, Eng& E, float2* phx, float2* phy, double* red) {
+__device__ __forceinline__ void process_position(const PosArgs& A, int k, int slot, bool first, Eng& E, float2* phx,
+                                                 float2* phy, double* red) {
     constexpr int N2 = N * N;
     const int t = threadIdx.x;
     const int L = A.L;
@@ -302,6 +305,7 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
         }
         const BufC ws(wsp, N2);
         const BufF g(A.gslab + ((size_t)slot * L + a) * N2, N2);
+        const BufF gin(A.gslab + ((size_t)slot * L + a) * N2, first ? 0u : N2);   // empty view: reads 0
         auto adjoint = [&](float2 v, float2 t, float2 w, float& gr) {
             const float2 b = cscale(v, inv_n2);
             const float2 tc = cconj(t);
@@ -326,7 +330,7 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
             else E.inv(NoOp(), post);
         } else {
             // accumulated into this workgroup's slab
-            auto post = staged<8, 16, true>([&](const Pt& p) { return AdjOp{ta[p.e], ws[p.e], g[p.e]}; },
+            auto post = staged<8, 16, true>([&](const Pt& p) { return AdjOp{ta[p.e], ws[p.e], gin[p.e]}; },
                                             [&](const Pt& p, float2 v, const AdjOp& o) {
                                                 float gr;
                                                 const float2 r = adjoint(v, o.t, o.w, gr);
@@ -360,7 +364,8 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
     if constexpr (KIND == K_BATCH) {
         if (want_probe) {
             const BufC ps(A.pslab + (size_t)slot * N2, N2);
-            E.each_R(staged<16>([&](const Pt& p) { return ps[p.e]; },
+            const BufC psin(A.pslab + (size_t)slot * N2, first ? 0u : N2);   // empty view: reads 0
+            E.each_R(staged<16>([&](const Pt& p) { return psin[p.e]; },
                                 [&](const Pt& p, float2& v, float2 o) { ps.put(p.e, cadd(o, cmulc(v, phase(p.y, p.x)))); }));
         }
     }
@@ -474,7 +479,7 @@ template <int N>
 __global__ __launch_bounds__(Cfg<N>::NT, Cfg<N>::MINB) void k_plain(PosArgs A, int k) {
     constexpr int NT = Cfg<N>::NT;
     MSET_LDS(N, NT)
-    process_position<N, NT, K_PLAIN>(A, k, 0, E, phx, phy, red);
+    process_position<N, NT, K_PLAIN>(A, k, 0, true, E, phx, phy, red);
 }
 
 // RC: the recompute variant (A.recompute), a separate instantiation so that the store variant carries
@@ -485,7 +490,7 @@ __global__ __launch_bounds__(Cfg<N>::NT, Cfg<N>::MINB) void k_batch(PosArgs A) {
     MSET_LDS(N, NT)
     for (int p = blockIdx.x; p < A.count; p += gridDim.x) {
         const int k = A.order ? A.order[A.first + p] : A.first + p;
-        process_position<N, NT, K_BATCH, RC>(A, k, blockIdx.x, E, phx, phy, red);
+        process_position<N, NT, K_BATCH, RC>(A, k, blockIdx.x, p == (int)blockIdx.x, E, phx, phy, red);
         __syncthreads();
     }
 }
@@ -496,7 +501,7 @@ __global__ __launch_bounds__(Cfg<N>::NT, Cfg<N>::MINB) void k_fwd(PosArgs A) {
     MSET_LDS(N, NT)
     for (int p = blockIdx.x; p < A.count; p += gridDim.x) {
         const int k = A.order ? A.order[A.first + p] : A.first + p;
-        process_position<N, NT, K_FWD>(A, k, blockIdx.x, E, phx, phy, red);
+        process_position<N, NT, K_FWD>(A, k, blockIdx.x, p == (int)blockIdx.x, E, phx, phy, red);
         __syncthreads();
     }
 }
