@@ -129,13 +129,20 @@ __global__ __launch_bounds__(NT) void k_axis_pass(PassArgs P) {
     __shared__ float2 tw[N];
     for (int i = threadIdx.x; i < N; i += NT) tw[i] = P.tw[i];
     const size_t sxy = (size_t)P.nx * P.ny;
-    // tile origin
+    // Tile origin.  Workgroups go to the 8 XCDs round-robin (XCD = blockIdx % 8); for the x and y passes
+    // the tiles are renumbered so that XCD c always works on the same eighth of the z planes, whose data
+    // then stays in that XCD's L2 from one pass to the next (measured: 15 % less rotation time at
+    // 128 x 128 x 64).  A z pass reads across all planes and keeps the tile order, which measured
+    // faster than splitting it by y.
+    constexpr int XCD = 8;
+    int b = blockIdx.x;
+    if (AXIS != 2 && gridDim.x % XCD == 0) b = (b % XCD) * (gridDim.x / XCD) + b / XCD;
     int row0 = 0, x0 = 0, fixed = 0;
-    if (AXIS == 0) row0 = blockIdx.x * TL;
+    if (AXIS == 0) row0 = b * TL;
     else {
         const int xt = (P.nx + TL - 1) / TL;
-        fixed = blockIdx.x / xt;            // z for AXIS 1, y for AXIS 2
-        x0 = (blockIdx.x % xt) * TL;
+        fixed = b / xt;                     // z for AXIS 1, y for AXIS 2
+        x0 = (b % xt) * TL;
     }
     const int nrows = P.ny * P.nz;
     auto gidx = [&](int l, int e, bool& ok, int& jx, int& jy, int& jz) -> size_t {
