@@ -95,10 +95,21 @@ template <int N, int NT_> struct TiledEngine {
                 if constexpr (std::is_same<Pre, NoOp>::value && !LOADG) v[r] = in[pos * N + ln];
                 else v[r] = pre(point(pos), LOADG ? make_float2(0.f, 0.f) : in[pos * N + ln]);
             }
-            if constexpr (D1 != NONE) dft<D1, false>(v, line, j);
-            if constexpr (!std::is_same<Mid, NoOp>::value) {
+            if constexpr (IsStaged<Mid>::value) {
+                // all 16 elements' operands are loaded before the first apply, so their latencies overlap
+                // (loading them ahead of D1 instead measured slower: the registers spill)
+                if constexpr (D1 != NONE) dft<D1, false>(v, line, j);
+                decltype(mid.load(Pt{})) op[R];
 #pragma unroll
-                for (int r = 0; r < R; ++r) v[r] = mid(point(j + R * r), v[r]);
+                for (int r = 0; r < R; ++r) op[r] = mid.load(point(j + R * r));
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[r] = mid.apply(point(j + R * r), v[r], op[r]);
+            } else {
+                if constexpr (D1 != NONE) dft<D1, false>(v, line, j);
+                if constexpr (!std::is_same<Mid, NoOp>::value) {
+#pragma unroll
+                    for (int r = 0; r < R; ++r) v[r] = mid(point(j + R * r), v[r]);
+                }
             }
             if constexpr (D2 != NONE) dft<D2, D1 != NONE>(v, line, j);
             if constexpr (SWAP) {
