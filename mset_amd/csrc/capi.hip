@@ -323,9 +323,17 @@ int issue_plain(mset_ctx* c) {
     CHK(c->ws->save.ensure((size_t)(c->L + 2) * plane(c), "save waves"));
     CHK(c->ws->gslab.ensure((size_t)c->L * plane(c), "gradient planes"));
     PosArgs A = base_args(c);
+    const float s1 = c->p.step_size[0];
     for (int k = 0; k < c->npos; ++k) {
-        Timed t(c, "position_plain");
-        HIPCHK(launch_plain(c->n, A, k, c->stream));
+        // transmission planes from the current volume, the position, then its object step
+        // (STEP02_FORWARD_4D.m:12-20, STEP03_BACK.m:84): the volume passes run on the whole device
+        HIPCHK(launch_transmission(c->V.p, c->ws->trans.p, c->n, c->nz, binning(c), sset(c) ? 1 : 0, c->p.sigma,
+                                   c->p.potential_pixelsize, c->stream));
+        {
+            Timed t(c, "position_plain");
+            HIPCHK(launch_plain(c->n, A, k, c->stream));
+        }
+        if (s1 != 0.f) HIPCHK(launch_update_volume(c->V.p, c->ws->gslab.p, c->n, c->nz, binning(c), s1, c->stream));
     }
     return MSET_OK;
 }
