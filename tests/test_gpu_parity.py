@@ -207,7 +207,7 @@ def test_bad_method_and_shapes_raise(mset):
 
 # ------------------------------------------------------------------ SSET ('run_sset' / 'error_sset')
 @pytest.mark.parametrize('n,nz,binning,mode', [(32, 6, 2, O.MODE_REF), (64, 5, 1, O.MODE_EXACT),
-                                               (128, 8, 4, O.MODE_REF)])
+                                               (128, 8, 4, O.MODE_REF), (256, 4, 2, O.MODE_EXACT)])
 def test_sset_sequential_and_error_vs_oracle(mset, n, nz, binning, mode):
     """STEP02_FORWARD_4D_SSET / STEP03_BACK_SSET with the sequential updates of Run_MSET_recon.m:134-158."""
     from mset_amd import lib
@@ -235,10 +235,11 @@ def test_sset_sequential_and_error_vs_oracle(mset, n, nz, binning, mode):
     assert np.abs(d - od).max() <= max(5e-3 * np.abs(od).max(), 4 * ulp)
 
 
-def test_sset_batched_step_is_the_fixed_state_gradient(mset):
+@pytest.mark.parametrize('n,nz,binning,npos', [(64, 6, 3, 7), (256, 4, 2, 5)])
+def test_sset_batched_step_is_the_fixed_state_gradient(mset, n, nz, binning, npos):
     """One mini-batch over every position: the object step is s1 * sum_k Re(grad_k) on every slice."""
     from mset_amd import lib
-    c = small_case(n=64, nz=6, binning=3, npos=7, seed=21, spread=8)
+    c = small_case(n=n, nz=nz, binning=binning, npos=npos, seed=21 + n, spread=n / 8)
     steps = (30.0, 0.0, 0.0)
     _upload_case(mset, c, steps=steps)
     e = mset.run_batched(c['npos'], method=lib.METHOD_SSET)
