@@ -162,6 +162,22 @@ template <int N, int NT_> struct TiledEngine {
         pass<false, INV, NONE, LOADG, true>(pre, NoOp());
     }
 
+    // col_first: real -> [pre, colF] -> half-forward (the start of a forward transform)
+    template <bool LOADG = false, class Pre = NoOp> __device__ __forceinline__ void col_first(const Pre& pre) {
+        pass<true, FWD, NONE, LOADG, true>(pre, NoOp());
+    }
+    // col_last: half-inverse -> [colI, post] -> real (the end of an inverse transform)
+    template <class Post> __device__ __forceinline__ void col_last(const Post& post) {
+        pass<true, INV, NONE, false, false>(NoOp(), post);
+    }
+    // Rebinds the row-major plane buffer, returning the previous one: the state held there stays put
+    // while the engine works on another plane (the recompute variant parks the adjoint wave this way).
+    __device__ __forceinline__ float2* swap_rs(float2* other) {
+        float2* old = RS;
+        RS = other;
+        return old;
+    }
+
     template <class F> __device__ __forceinline__ void each_R(const F& f) {
         for (int e = threadIdx.x; e < N * N; e += NT) {
             const int y = e % N, x = e / N;

@@ -126,8 +126,8 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
     const bool multi = !A.sset && L > 0;
     // An engine whose plane lives in global memory fuses the passes that meet between transforms
     // (row_step, col_step): from here to the end of the adjoint the plane alternates between the
-    // half-forward and half-inverse states.  The recompute variant keeps the plain transforms.
-    constexpr bool FUSE = Eng::FUSED && !recompute;
+    // half-forward and half-inverse states.
+    constexpr bool FUSE = Eng::FUSED;
     auto psi0_post = staged<16, 32>([&](const Pt& p) { return tr0[p.e]; }, [&](const Pt& p, float2 v, float2 o) {
         float2 w = cscale(v, inv_n2);
         if (KIND != K_FWD) sv0.put(p.e, w);
@@ -236,7 +236,23 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
         const float2* wsp = save + (size_t)((exact || A.sset) ? a : a + 1) * N2;
         const bool rebuild = recompute && a > 0;
         if (recompute) wsp = exact ? (a > 0 ? prev : save) : cur;
-        if constexpr (KIND == K_BATCH) {
+        if constexpr (KIND == K_BATCH && FUSE) {
+            if (rebuild) {
+                // the adjoint wave (half-inverse state) stays in its buffer while the engine rebuilds psi_a
+                // into prev on the park plane: three fused passes, no copies
+                float2* held = E.swap_rs(save + 4 * (size_t)N2);
+                const BufC cb(cur, N2), pb(prev, N2);
+                E.template col_first<true>(staged<16>([&](const Pt& p) { return cb[p.e]; }, [&](const Pt&, float2, float2 o) { return o; }));
+                E.row_step(staged<16, 32>([&](const Pt& p) { return pTb[p.e]; },
+                                          [&](const Pt&, float2 v, float2 o) { return cmulc(v, o); }));
+                E.col_last(staged<16, 32>([&](const Pt& p) { return ta[p.e]; }, [&](const Pt& p, float2 v, float2 o) {
+                    const float2 w = cmulc(cscale(v, inv_n2), o);
+                    pb.put(p.e, w);
+                    return w;
+                }));
+                E.swap_rs(held);
+            }
+        } else if constexpr (KIND == K_BATCH) {
             if (rebuild) {
                 // park the adjoint wave (Fourier domain), rebuild psi_a into prev
                 E.each_R([&](const Pt& p, float2& v) { park.put(p.e, v); });
