@@ -198,6 +198,13 @@ int mset_recon_profile_enable(mset_recon* r, int enable);
 int mset_recon_profile_read(mset_recon* r, const char* name, int* launches, double* total_ms);
 void* mset_recon_stream(mset_recon* r);
 
+/* ---- page-locked host buffers ------------------------------------------------------------------- */
+/* Host memory for the arrays a caller uploads (patterns, volumes): copies from it run at the full link
+ * rate instead of through the driver's staging of pageable memory.  No counterpart in the reference,
+ * whose MATLAB arrays are pageable; any host pointer stays valid for every entry point. */
+int mset_host_alloc(size_t bytes, void** ptr);
+int mset_host_free(void* ptr);
+
 /* ---- device capacity (admission control for several reconstructions on one device) ------------- */
 /* Free and total device memory in bytes. */
 int mset_device_memory(int device, size_t* free_bytes, size_t* total_bytes);

@@ -768,6 +768,24 @@ int mset_device_memory(int device, size_t* free_bytes, size_t* total_bytes) {
     return MSET_OK;
 }
 
+int mset_host_alloc(size_t bytes, void** ptr) {
+    if (!ptr) return fail(MSET_EINVAL, "null output");
+    *ptr = nullptr;
+    if (bytes == 0) return fail(MSET_EINVAL, "zero-byte host buffer");
+    hipError_t e = hipHostMalloc(ptr, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        return fail(MSET_ENOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+    }
+    return MSET_OK;
+}
+
+int mset_host_free(void* ptr) {
+    if (!ptr) return MSET_OK;
+    HIPCHK(hipHostFree(ptr));
+    return MSET_OK;
+}
+
 int mset_batch_slots(int device, int n, int* slots) {
     if (!slots) return fail(MSET_EINVAL, "null output");
     if (!plane_size_supported(n)) return fail(MSET_EUNSUPPORTED, "plane size not built");

@@ -26,7 +26,7 @@ EXPORTS = ['mset_create', 'mset_destroy', 'mset_last_error', 'mset_stream', 'mse
            'mset_recon_snapshot', 'mset_recon_restore', 'mset_recon_profile_enable', 'mset_recon_profile_read',
            'mset_recon_stream', 'mset_tv', 'mset_recon_tv', 'mset_device_memory', 'mset_batch_slots',
            'mset_set_exit_waves', 'mset_exit_waves', 'mset_recon_set_exit_waves', 'mset_set_slot_limit',
-           'mset_recon_set_slot_limit']
+           'mset_recon_set_slot_limit', 'mset_host_alloc', 'mset_host_free']
 PATH_PLAIN, PATH_BATCHED = 0, 1
 EXIT_WAVES_AUTO, EXIT_WAVES_STORE, EXIT_WAVES_RECOMPUTE = 0, 1, 2
 
@@ -81,7 +81,8 @@ def load():
         'mset_device_memory': ([I, P, P], I), 'mset_batch_slots': ([I, I, P], I),
         'mset_set_exit_waves': ([P, I], I), 'mset_exit_waves': ([P, P, P], I),
         'mset_recon_set_exit_waves': ([P, I], I), 'mset_set_slot_limit': ([P, I], I),
-        'mset_recon_set_slot_limit': ([P, I], I),
+        'mset_recon_set_slot_limit': ([P, I], I), 'mset_host_alloc': ([C.c_size_t, PP], I),
+        'mset_host_free': ([P], I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -105,6 +106,30 @@ def batch_slots(n, device=0):
     s = C.c_int(0)
     _chk(L.mset_batch_slots(int(device), int(n), C.addressof(s)))
     return int(s.value)
+
+
+class PinnedArray:
+    """A numpy array over page-locked host memory (mset_host_alloc); freed by free() or when collected.
+    Upload from `.array` like from any array: the copy to the device then runs at the full link rate."""
+
+    def __init__(self, shape, dtype=np.float32):
+        self._L = load()
+        dt = np.dtype(dtype)
+        n = int(np.prod(shape)) * dt.itemsize
+        p = C.c_void_p()
+        _chk(self._L.mset_host_alloc(C.c_size_t(max(n, 1)), C.byref(p)))
+        self._p = p
+        buf = (C.c_char * max(n, 1)).from_address(p.value)
+        self.array = np.frombuffer(buf, dtype=dt, count=int(np.prod(shape))).reshape(shape)
+
+    def free(self):
+        if getattr(self, '_p', None) is not None and self._p.value:
+            self.array = None
+            self._L.mset_host_free(self._p)
+            self._p = None
+
+    def __del__(self):
+        self.free()
 
 
 def _ptr(a):

@@ -275,3 +275,22 @@ def test_single_plane_and_single_position_edges(mset, n, nz, binning, npos, mode
     assert abs(e2 / oe - 1) < LOSS_TOL
     assert_update_close(V2, oV, c['V'], npos)
     assert_update_close(p2, op, c['probe'], npos)
+
+
+def test_upload_from_page_locked_host_memory(mset):
+    """Patterns staged in mset_host_alloc memory upload to the same device state as pageable ones."""
+    from mset_amd import lib
+    c = small_case(n=64, nz=8, binning=2, npos=9, seed=31, spread=8.0)
+    _upload_case(mset, c)
+    ref = mset.grad()
+    pin = lib.PinnedArray(c['M'].shape, np.float32)
+    try:
+        pin.array[...] = c['M']
+        mset.upload(c['V'], c['probe'], c['prop'], c['xl'], c['yl'], pin.array, c['sig'], c['dx'], slice_binning=c['bin'],
+                    n_scan=(c['npos'], 1))
+        got = mset.grad()
+    finally:
+        pin.free()
+    for a, b in zip(got[:3], ref[:3]):
+        assert np.array_equal(a, b)
+    assert got[3] == ref[3]
