@@ -109,8 +109,9 @@ def batch_slots(n, device=0):
 
 
 class PinnedArray:
-    """A numpy array over page-locked host memory (mset_host_alloc); freed by free() or when collected.
-    Upload from `.array` like from any array: the copy to the device then runs at the full link rate."""
+    """A numpy array over page-locked host memory (mset_host_alloc), released when neither this object nor
+    any view of `.array` is reachable, or at once by free() (after which no view may be used).  Upload
+    from `.array` like from any array: the copy to the device then runs at the full link rate."""
 
     def __init__(self, shape, dtype=np.float32):
         self._L = load()
@@ -120,6 +121,7 @@ class PinnedArray:
         _chk(self._L.mset_host_alloc(C.c_size_t(max(n, 1)), C.byref(p)))
         self._p = p
         buf = (C.c_char * max(n, 1)).from_address(p.value)
+        buf._owner = self            # views of the array keep the allocation alive
         self.array = np.frombuffer(buf, dtype=dt, count=int(np.prod(shape))).reshape(shape)
 
     def free(self):
