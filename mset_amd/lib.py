@@ -7,6 +7,7 @@ measured data is float32 [num_scan_pos, ny, nx] in MATLAB's linear cell order.
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -109,29 +110,31 @@ def batch_slots(n, device=0):
 
 
 class PinnedArray:
-    """A numpy array over page-locked host memory (mset_host_alloc), released when neither this object nor
-    any view of `.array` is reachable, or at once by free() (after which no view may be used).  Upload
+    """A numpy array over page-locked host memory (mset_host_alloc), released as soon as neither this object
+    nor any view of `.array` is reachable, or at once by free() (after which no view may be used).  Upload
     from `.array` like from any array: the copy to the device then runs at the full link rate."""
 
     def __init__(self, shape, dtype=np.float32):
-        self._L = load()
+        L = load()
         dt = np.dtype(dtype)
         n = int(np.prod(shape)) * dt.itemsize
         p = C.c_void_p()
-        _chk(self._L.mset_host_alloc(C.c_size_t(max(n, 1)), C.byref(p)))
-        self._p = p
+        _chk(L.mset_host_alloc(C.c_size_t(max(n, 1)), C.byref(p)))
         buf = (C.c_char * max(n, 1)).from_address(p.value)
-        buf._owner = self            # views of the array keep the allocation alive
+        # the release hangs off the ctypes buffer that every view of the array keeps alive through its
+        # base chain; nothing on that chain refers back to this object, so reference counting frees the
+        # memory when the last view goes (no wait for the cycle collector)
+        self._fin = weakref.finalize(buf, L.mset_host_free, C.c_void_p(p.value))
         self.array = np.frombuffer(buf, dtype=dt, count=int(np.prod(shape))).reshape(shape)
 
-    def free(self):
-        if getattr(self, '_p', None) is not None and self._p.value:
-            self.array = None
-            self._L.mset_host_free(self._p)
-            self._p = None
+    @property
+    def alive(self):
+        """True until the page-locked memory has been released."""
+        return self._fin.alive
 
-    def __del__(self):
-        self.free()
+    def free(self):
+        self.array = None
+        self._fin()
 
 
 def _ptr(a):

@@ -50,3 +50,49 @@ def test_build_script_flags():
     from mset_amd import build
     assert '--offload-arch=gfx950' in build.FLAGS
     assert '-packed-fp32-ops' in build.FLAGS
+
+
+class _FakeHostAlloc:
+    """Stands in for mset_host_alloc / mset_host_free (which need a device) to test PinnedArray's lifetime."""
+
+    def __init__(self):
+        self.bufs, self.freed = {}, []
+
+    def mset_host_alloc(self, size, out):
+        b = C.create_string_buffer(size.value)
+        C.cast(out, C.POINTER(C.c_void_p))[0] = C.addressof(b)
+        self.bufs[C.addressof(b)] = b
+        return 0
+
+    def mset_host_free(self, ptr):
+        self.freed.append(ptr.value)
+        return 0
+
+
+def test_pinned_array_is_released_with_its_last_view(monkeypatch):
+    import gc
+
+    import numpy as np
+
+    from mset_amd import lib
+    fake = _FakeHostAlloc()
+    monkeypatch.setattr(lib, 'load', lambda: fake)
+    gc.disable()                        # the release must not depend on the cycle collector
+    try:
+        pa = lib.PinnedArray((4, 8), np.float32)
+        addr = pa.array.ctypes.data
+        view = pa.array[1:]
+        view[...] = 3.0
+        del pa
+        assert fake.freed == []         # a view still refers to the memory
+        assert float(view.sum()) == 3.0 * 24
+        del view
+        assert fake.freed == [addr]
+        pb = lib.PinnedArray((2,), np.complex64)
+        b = pb.array.ctypes.data
+        pb.free()                       # explicit release, exactly once
+        assert not pb.alive and fake.freed == [addr, b]
+        del pb
+        assert fake.freed == [addr, b]
+    finally:
+        gc.enable()

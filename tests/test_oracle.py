@@ -371,6 +371,71 @@ def test_run_tilt_single_matches_double():
     assert np.abs(xf - xd).max() < 1e-4 and np.abs(yf - yd).max() < 1e-4
 
 
+def _sequential_by_hand(c, steps, mode, stale=False):
+    """Run_MSET_recon.m:134-158 spelled out as a loop of one-position gradients: for each position k the
+    gradient is taken at the state left by positions 0..k-1, then V += s1 Re(grad) on every voxel of each
+    bin (STEP03_BACK.m:17-21), probe -= s2 shift(bw, -d) (STEP03_BACK.m:29-31) and (row, col) += s3 terms
+    (STEP03_BACK.m:35-38).  stale: every gradient at the initial state (the state never advances)."""
+    V = np.array(c['V'], np.float64)
+    pr = np.array(c['probe'], np.complex128)
+    xl = np.array(c['xl'], np.float32)
+    yl = np.array(c['yl'], np.float32)
+    V0, p0 = V.copy(), pr.copy()
+    zbin = np.arange(V.shape[0]) // c['bin']
+    for k in range(len(xl)):
+        dV, dp, dpos, _ = O.grad_fixed(V0 if stale else V, p0 if stale else pr, c['prop'], c['bprop'],
+                                       xl[k:k + 1].astype(np.float64), yl[k:k + 1].astype(np.float64),
+                                       c['M'][k:k + 1], c['bin'], c['sig'], c['dx'], mode=mode)
+        V = V + steps[0] * dV[zbin]
+        pr = pr - steps[1] * dp
+        xl[k] = np.float32(np.float64(xl[k]) + steps[2] * dpos[0, 0])
+        yl[k] = np.float32(np.float64(yl[k]) + steps[2] * dpos[0, 1])
+    return V, pr, xl, yl
+
+
+@pytest.mark.parametrize('mode', [O.MODE_REF, O.MODE_EXACT])
+def test_run_tilt_is_the_sequential_composition_of_one_position_gradients(mode):
+    """Pins the update order of oracle run_tilt: after each position the volume, the probe and that
+    position are advanced before the next position's forward pass.  A loop that did not advance the state,
+    or applied a step before taking the gradient, fails the first comparison; the stale loop shows the
+    steps are large enough for that to be visible."""
+    c = _small_case(npos=5, seed=11)
+    steps = [40.0, 2e-3, 5e-2]
+    V, pr, xl, yl, _, _ = O.run_tilt(c['V'], c['probe'], c['prop'], c['bprop'], c['xl'], c['yl'], c['M'], 5, 1,
+                                     c['bin'], c['sig'], c['dx'], steps, mode=mode)
+    hV, hp, hx, hy = _sequential_by_hand(c, steps, mode)
+    assert rel(V - c['V'], hV - c['V']) < 1e-10 and rel(pr - c['probe'], hp - c['probe']) < 1e-10
+    assert np.array_equal(xl, hx) and np.array_equal(yl, hy)
+    sV, sp, sx, _ = _sequential_by_hand(c, steps, mode, stale=True)
+    assert rel(sV - c['V'], V - c['V']) > 1e-3 and rel(sp - c['probe'], pr - c['probe']) > 1e-3
+    x0 = np.asarray(c['xl'], np.float64)
+    assert rel(sx - x0, xl - x0) > 1e-3
+
+
+def test_reference_position_term_uses_matlab_gradient_edges():
+    """STEP03_BACK.m:36-38: [de_probey, de_probex] = gradient(conj(psi0)); row += Re sum(de_probex .* bw),
+    col += Re sum(de_probey .* bw).  numpy.gradient with edge_order=1 is MATLAB's gradient (central
+    difference inside, one-sided first difference on the first and last sample).  A white-spectrum
+    incident wave puts energy on the field border, so a wrap-around or zeroed edge rule fails."""
+    c = _small_case(n=16, nz=4, binning=2, npos=1, seed=4)
+    n = c['n']
+    rng = np.random.default_rng(21)
+    psi0 = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) * 3.0
+    save, _, resi, _ = O.forward(c['V'], psi0, c['prop'], c['M'][0], c['bin'], c['sig'], c['dx'])
+    _, bw, dpos = O.back(c['V'], save, psi0, resi, c['bprop'], c['bin'], c['sig'], c['dx'], 9.0, 9.0,
+                         mode=O.MODE_REF)
+    cp = np.conj(psi0)                                  # [y][x]: x (MATLAB dim 1, row) is the last axis
+    gx = np.gradient(cp, axis=1, edge_order=1)
+    gy = np.gradient(cp, axis=0, edge_order=1)
+    want = np.array([np.real(np.sum(gx * bw)), np.real(np.sum(gy * bw))])
+    assert np.abs(dpos - want).max() <= 1e-12 * np.abs(want).max(), (dpos, want)
+    # the same sums with a periodic central difference differ at this level
+    px = (np.roll(cp, -1, axis=1) - np.roll(cp, 1, axis=1)) / 2
+    py = (np.roll(cp, -1, axis=0) - np.roll(cp, 1, axis=0)) / 2
+    wrap = np.array([np.real(np.sum(px * bw)), np.real(np.sum(py * bw))])
+    assert np.abs(wrap - want).max() > 0.05 * np.abs(want).max()
+
+
 def test_run_tilt_with_zero_steps_is_the_error_pass():
     c = _small_case(npos=4)
     V, p, xl, yl, e, ep = O.run_tilt(c['V'], c['probe'], c['prop'], c['bprop'], c['xl'], c['yl'], c['M'], 2, 2,
