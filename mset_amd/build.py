@@ -10,6 +10,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 SOURCES = ['multislice.hip', 'elementwise.hip', 'rotate.hip', 'tv.hip', 'capi.hip']
 LIB = os.path.join(HERE, 'libmset_amd.so')
+# stand-alone counter-calibration program (known-size copies for rocprofv3's FETCH_SIZE / WRITE_SIZE)
+CALIBRATE = os.path.join(HERE, 'mset_calibrate')
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 ARCH = os.environ.get('MSET_OFFLOAD_ARCH', 'gfx950')
 # packed FP32 instructions are disabled: some of their forms have produced wrong results on this hardware
@@ -42,8 +44,14 @@ def build(verbose=True):
     if r.returncode != 0:
         raise RuntimeError('link failed:\n' + r.stdout + r.stderr)
     os.replace(tmp, LIB)
+    tmp = CALIBRATE + '.tmp'
+    cmd = [HIPCC] + FLAGS + ['-o', tmp, os.path.join(HERE, 'csrc', 'calibrate.hip')]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError('hipcc failed for calibrate.hip:\n' + r.stdout + r.stderr)
+    os.replace(tmp, CALIBRATE)
     if verbose:
-        print('built', LIB)
+        print('built', LIB, 'and', CALIBRATE)
     return LIB
 
 
