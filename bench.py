@@ -1,17 +1,22 @@
 """Benchmark of the multislice forward + adjoint path on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--workload C2|C3] [--batch B] [--no-autotune] [--gpus 1]
+    python bench.py [--steps K] [--warmup W] [--workload C3|C2] [--batch B] [--no-autotune] [--gpus 1]
 
-Workload: the GPU configuration of the reference's plain method (64^3 volume at 0.4 A in a 128 x 128
-field, probe 128 x 128 at 300 kV / 21 mrad, 32 x 32 scan with sub-pixel jitter, 11 tilts from -50 to
-+50 deg, 1.6 A slices -> 16 propagation planes, Poisson data at 5e4 e/A^2), synthetic and seeded.
-One step = one full update pass of the reconstruction over all 11 tilts: for each tilt the inverse
-Fourier-shear rotation of the volume, forward + adjoint for all 1024 positions with the object update,
-the forward rotation and positivity (Run_MSET_recon.m:119-187).  Inputs are resident in HBM before
-timing starts.  The headline path runs positions in mini-batches of B per launch (one workgroup per
-position, up to one per CU in flight) replayed from a captured hipGraph; B and the persistent grid are
-picked by the autotuner (mset_amd/autotune.py) before timing.  The plain path (one position per launch,
-sequential updates, the reference's semantics) is measured on the same workload and reported beside it.
+Default workload (C3, the largest single-GPU configuration): a 128^3 Gaussian-atom volume at 0.4 A in a
+256 x 256 field (256 x 256 x 128 voxels), probe 256 x 256 at 300 kV / 21 mrad, 21 tilts from -60 to +60
+deg, each tilt with its own square scan of side drawn from 24..48 (2 px steps + U[-0.5, 0.5] px jitter),
+1.6 A slices -> 32 propagation planes, Poisson data at 5e4 e/A^2; synthetic and seeded.  `--workload C2`
+runs the 128 x 128 configuration (64^3 volume, 32 x 32 scans, 11 tilts, 16 planes).
+
+One step = one full update pass of the reconstruction over every tilt (Run_MSET_recon.m:119-187): for
+each tilt the inverse Fourier-shear rotation of the volume, forward + adjoint for all of the tilt's
+positions with the object, probe and position updates, the forward rotation and positivity.  The step
+sizes form all three gradient terms on every position, as the reference does (STEP03_BACK.m:26-38); the
+object-only pass of the reference's example (main.m:42, step [100, 0, 0]) is timed and reported beside
+it.  Inputs are resident in HBM before timing starts.  The headline path runs positions in mini-batches
+of B per launch on a persistent grid, replayed from a captured hipGraph; B and the grid are picked by the
+autotuner (mset_amd/autotune.py) before timing.  The plain path (one position per launch, sequential
+updates, the reference's semantics) is timed on one tilt of the same workload and reported beside it.
 
 Prints one JSON line.
 """
@@ -36,47 +41,68 @@ WORKLOADS = {
           'each tilt a square scan of side U{24..48} (2 px + U[-0.5,0.5]), 32 planes (1.6 A), one update pass '
           'over all tilts per step',
 }
+# object, probe and position step sizes of the timed pass.  The object step is the example's (main.m:42);
+# the probe and position steps move the probe by about a percent and the positions by about 0.01 px per
+# pass on these workloads.  Nonzero, they make the kernels form the probe and position terms.
+STEPS_ALL = (100.0, 0.02, 1e8)
+STEPS_OBJECT = (100.0, 0.0, 0.0)
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(wl, positions, threads):
+def host_threads():
+    """Cores this process may use: its CPU affinity, capped by OMP_NUM_THREADS when that is set."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    omp = os.environ.get('OMP_NUM_THREADS')
+    if omp and omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
+
+
+def cpu_baseline(wl, tilt, positions, threads, steps):
     """The oracle's single-precision restatement of the MATLAB CPU path (Run_MSET_recon.m:134-158 with
-    STEP02_FORWARD_4D / STEP03_BACK: sequential positions, whole-volume transmission per position),
-    timed on a bounded sample of the same workload."""
+    STEP02_FORWARD_4D / STEP03_BACK: sequential positions, whole-volume transmission per position), timed
+    on `positions` positions of one tilt with `threads` OpenMP threads."""
     from oracle import oracle as O
     used = O.set_threads(threads)
     bprop = np.conj(wl.prop)
     V0 = wl.truth * 0.0
     t0 = time.perf_counter()
-    O.run_tilt(V0, wl.probe, wl.prop, bprop, wl.xlists[0][:positions], wl.ylists[0][:positions],
-               wl.measured[0][:positions], positions, 1, wl.binning, wl.sigma, wl.dx, (100.0, 0.0, 0.0), single=True)
+    O.run_tilt(V0, wl.probe, wl.prop, bprop, wl.xlists[tilt][:positions], wl.ylists[tilt][:positions],
+               wl.measured[tilt][:positions], positions, 1, wl.binning, wl.sigma, wl.dx, steps, single=True)
     dt = time.perf_counter() - t0
-    return dict(value=positions * wl.L / dt, unit=UNIT, cores=used, kind='port',
-                sample='%d positions of tilt 1 (%d planes each), sequential updates, FP32, %.1f s' %
-                       (positions, wl.L, dt))
+    return dict(value=positions * wl.L / dt, unit=UNIT, cores=used, seconds=dt,
+                sample='%d positions of tilt %d (%d planes each), sequential updates, FP32, %d thread(s), %.1f s'
+                       % (positions, tilt + 1, wl.L, used, dt))
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=5)
+    ap.add_argument('--steps', type=int, default=10)
     ap.add_argument('--warmup', type=int, default=2)
-    ap.add_argument('--workload', default='C2', choices=['C2', 'C3'],
-                    help='C2 (default): the configuration the metric is quoted on; C3: 256^2 probe, uneven scans')
+    ap.add_argument('--workload', default='C3', choices=['C2', 'C3'],
+                    help='C3 (default): 256^2 probe, 21 tilts with uneven scans; C2: 128^2 probe, 11 tilts')
     ap.add_argument('--batch', type=int, default=None,
-                    help='positions per mini-batch (default: picked by the autotuner from 256..1024 for C2, '
-                         '64..512 for C3)')
+                    help='positions per mini-batch (default: picked by the autotuner from 512..1024 for C3, '
+                         '512..1024 for C2)')
     ap.add_argument('--no-autotune', action='store_true',
-                    help='skip the autotuner: batch 1024 (C2) / 512 (C3) on every resident workgroup')
+                    help='skip the autotuner: batch 512 (C3) / 1024 (C2) on every resident workgroup')
     ap.add_argument('--no-graph', action='store_true')
-    ap.add_argument('--plain-tilts', type=int, default=None, help='tilts of the plain path to time (0 = skip; '
-                    'default 1 for C2, 0 for C3)')
+    ap.add_argument('--object-only-steps', type=int, default=3,
+                    help='timed passes with the object step only (0 = skip)')
+    ap.add_argument('--plain-tilts', type=int, default=1, help='tilts of the plain path to time (0 = skip); the '
+                    'smallest tilts are taken')
     ap.add_argument('--cpu-positions', type=int, default=None,
-                    help='positions of the CPU sample (0 = skip; default one full tilt for C2, 32 for C3)')
-    ap.add_argument('--cpu-threads', type=int, default=min(16, os.cpu_count() or 1))
+                    help='positions of the all-cores CPU sample (0 = skip both CPU samples; default 48 for C3, '
+                         '256 for C2); the one-thread sample takes a third of them')
+    ap.add_argument('--cpu-threads', type=int, default=None, help='threads of the all-cores sample (default: the '
+                    'cores this process may use)')
     args = ap.parse_args()
     world = int(os.environ.get('WORLD_SIZE', '1'))
     if args.gpus != 1 or world != 1:
@@ -84,19 +110,19 @@ def main():
               'run with --gpus 1', file=sys.stderr)
         sys.exit(2)
 
-    from mset_amd import lib, synth
+    from mset_amd import lib, physics, synth
     big = args.workload == 'C3'
-    batches = (64, 128, 256, 512) if big else (256, 512, 1024)
+    batches = (512, 1024)
     if args.batch is not None:
         batches = (args.batch,)
     elif args.no_autotune:
-        batches = (max(batches),)
-    if args.plain_tilts is None:
-        args.plain_tilts = 0 if big else 1
+        batches = (512 if big else 1024,)
     if args.cpu_positions is None:
-        args.cpu_positions = 32 if big else 1024
+        args.cpu_positions = 48 if big else 256
+    if args.cpu_threads is None:
+        args.cpu_threads = host_threads()
 
-    steps = (100.0, 0.0, 0.0)     # object step only, as the reference's example configuration (main.m:42)
+    t_setup = time.perf_counter()
     log('generating workload %s ...' % args.workload)
     wl = synth.Workload(args.workload)
     ctx = lib.Context(0)
@@ -105,37 +131,43 @@ def main():
     ntilt = len(wl.angles)
     rec0 = np.zeros_like(wl.truth)
     mean_I = sum(float(m.sum()) for m in wl.measured) / wl.positions()
-    from mset_amd import physics
     probe0 = physics.normalise_probe(wl.probe.astype(np.complex128), mean_I).astype(np.complex64)
+    log('workload ready: %d tilts, %d positions, %.1f s' % (ntilt, wl.positions(), time.perf_counter() - t_setup))
 
-    def make_recon():
-        r = lib.Recon(rec0, wl.prop, ntilt, wl.sigma, wl.dx, slice_binning=wl.binning, step_size=steps)
-        for j, ang in enumerate(wl.angles):
-            r.set_tilt(j, ang, probe0, wl.xlists[j], wl.ylists[j], wl.measured[j], n_scan=wl.n_scans[j])
+    def make_recon(tilts, steps):
+        r = lib.Recon(rec0, wl.prop, len(tilts), wl.sigma, wl.dx, slice_binning=wl.binning, step_size=steps)
+        for i, j in enumerate(tilts):
+            r.set_tilt(i, wl.angles[j], probe0, wl.xlists[j], wl.ylists[j], wl.measured[j], n_scan=wl.n_scans[j])
         return r
 
     units = wl.pos_slices()
     use_graph = not args.no_graph
-    r = make_recon()
+    r = make_recon(range(ntilt), STEPS_ALL)
     tuned = None
     if not args.no_autotune:
         from mset_amd import autotune
-        tuned = autotune.tune_recon(r, batches=batches, log=log)
+        full = lib.batch_slots(wl.P)
+        tuned = autotune.tune_recon(r, batches=batches, slots=[full, full // 2], log=log)
         args.batch = tuned['batch']
         tuned = dict(batch=tuned['batch'], slots=tuned['slots'],
                      pass_ms={'%d/%d' % (sl, b): round(dt, 2) for sl, b, dt, _ in tuned['table']})
     else:
         args.batch = batches[0]
-    log('batched path: batch %d, graph %s' % (args.batch, use_graph))
-    for _ in range(args.warmup):
-        r.iterate(lib.PATH_BATCHED, args.batch, use_graph=use_graph, want_error=False)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        r.iterate(lib.PATH_BATCHED, args.batch, use_graph=use_graph, want_error=False)
-    dt = time.perf_counter() - t0
+    log('batched path: batch %d, graph %s, steps %s' % (args.batch, use_graph, STEPS_ALL))
+
+    def timed(nsteps, nwarm):
+        for _ in range(nwarm):
+            r.iterate(lib.PATH_BATCHED, args.batch, use_graph=use_graph, want_error=False)
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            r.iterate(lib.PATH_BATCHED, args.batch, use_graph=use_graph, want_error=False)
+        return time.perf_counter() - t0
+
+    dt = timed(args.steps, args.warmup)
     ms_step = dt / args.steps * 1e3
     value = units * args.steps / dt
     err_after = float(np.mean(r.error_pass()))
+    log('timed: %.2f ms per step, %.0f pos-slices/s' % (ms_step, value))
 
     # dominant kernel timed with HIP events on the reconstruction's stream (no graph while profiling)
     r.profile(True)
@@ -143,7 +175,15 @@ def main():
     nb, ms_b = r.profile_read('position_batch')
     nr, ms_r = r.profile_read('rotate')
     r.profile(False)
+
+    object_only = None
+    if args.object_only_steps > 0:
+        r.set_steps(STEPS_OBJECT)
+        dto = timed(args.object_only_steps, 1)
+        object_only = dict(step_size=list(STEPS_OBJECT), steps=args.object_only_steps,
+                           ms_per_step=dto / args.object_only_steps * 1e3, value=units * args.object_only_steps / dto)
     r.close()
+
     per_launch_units = units / nb
     avg_ms = ms_b / nb
     b_alg = 32.0 * wl.P * wl.P            # bytes per probe-position-slice (store-exit-wave formulation)
@@ -156,24 +196,28 @@ def main():
 
     plain = None
     if args.plain_tilts > 0:
-        log('plain path: %d tilt(s)' % args.plain_tilts)
-        rp = lib.Recon(rec0, wl.prop, args.plain_tilts, wl.sigma, wl.dx, slice_binning=wl.binning, step_size=steps)
-        for j in range(args.plain_tilts):
-            rp.set_tilt(j, wl.angles[j], probe0, wl.xlists[j], wl.ylists[j], wl.measured[j], n_scan=wl.n_scans[j])
+        small = sorted(range(ntilt), key=lambda j: wl.npos_of(j))[:args.plain_tilts]
+        log('plain path: tilt(s) %s' % [j + 1 for j in small])
+        rp = make_recon(small, STEPS_ALL)
         rp.profile(True)
         t0 = time.perf_counter()
         rp.iterate(lib.PATH_PLAIN, 1, use_graph=False, want_error=False)
         dtp = time.perf_counter() - t0
         npl, ms_pl = rp.profile_read('position_plain')
         rp.close()
-        pvalue = sum(wl.npos_of(j) for j in range(args.plain_tilts)) * wl.L / dtp
-        plain = dict(value=pvalue, unit=UNIT, tilts_timed=args.plain_tilts,
+        pvalue = sum(wl.npos_of(j) for j in small) * wl.L / dtp
+        plain = dict(value=pvalue, unit=UNIT, tilts_timed=[j + 1 for j in small], step_size=list(STEPS_ALL),
                      kernel_avg_ms=ms_pl / max(npl, 1), speedup_of_batched=value / pvalue)
 
     cpu = None
     if args.cpu_positions > 0:
-        log('cpu baseline: %d positions' % args.cpu_positions)
-        cpu = cpu_baseline(wl, args.cpu_positions, args.cpu_threads)
+        tilt = ntilt // 2
+        one = max(1, args.cpu_positions // 3)
+        log('cpu baseline: %d positions on 1 thread, %d on %d threads' % (one, args.cpu_positions, args.cpu_threads))
+        c1 = cpu_baseline(wl, tilt, one, 1, STEPS_ALL)
+        ca = cpu_baseline(wl, tilt, args.cpu_positions, args.cpu_threads, STEPS_ALL)
+        cpu = dict(value=ca['value'], unit=UNIT, cores=ca['cores'], kind='port', sample=ca['sample'],
+                   nproc=os.cpu_count(), single_thread=dict(value=c1['value'], cores=1, sample=c1['sample']))
 
     out = dict(
         metric=METRIC, value=value, unit=UNIT, n_gpus=1, steps=args.steps, warmup=args.warmup,
@@ -181,12 +225,12 @@ def main():
         data='synthetic: seeded FCC Gaussian-atom potential, device-simulated Poisson patterns',
         config=dict(workload=WORKLOADS[args.workload],
                     path='batched', batch=args.batch, hip_graph=use_graph, positions_per_step=wl.positions(),
-                    planes=wl.L, pos_slices_per_step=units, step_size=list(steps), autotune=tuned,
-                    error_after=err_after, rotate_ms_per_step=ms_r),
+                    tilts=ntilt, planes=wl.L, probe=wl.P, pos_slices_per_step=units, step_size=list(STEPS_ALL),
+                    autotune=tuned, error_after=err_after, rotate_ms_per_step=ms_r),
         roofline=dict(bound='hbm', achieved=achieved, peak=HBM_PEAK_GBS, unit='GB/s', frac=achieved / HBM_PEAK_GBS,
-                      traffic=traffic, kernel='position_batch', kernel_avg_ms=avg_ms,
+                      traffic=traffic, kernel='position_batch (k_batch<%d>)' % wl.P, kernel_avg_ms=avg_ms,
                       units_per_launch=per_launch_units, bytes_per_unit=b_alg),
-        cpu_baseline=cpu, plain=plain,
+        cpu_baseline=cpu, plain=plain, object_only=object_only,
     )
     print(json.dumps(out))
 

@@ -56,6 +56,8 @@ typedef struct mset_params {
     float potential_pixelsize;    /* dx, A */
     float step_size[3];           /* object, probe, position step sizes */
     int32_t grad_mode;            /* MSET_GRAD_REF (reference semantics) or MSET_GRAD_EXACT */
+    int32_t diffraction_transpose; /* STEM_data.diffraction_transpose (STEP00_INIT.m:58-73): nonzero transposes
+                                      every measured pattern at ingest, as MATLAB's transpose() of each cell */
 } mset_params;
 
 /* ---- lifetime ------------------------------------------------------------------------------- */
@@ -103,6 +105,7 @@ int mset_set_state(mset_ctx* ctx, const float* RVol, const float* probe, const f
  *   error  sum of the positions' error terms, normalised as in mset_run. */
 int mset_grad(mset_ctx* ctx, int first, int count, float* dV, float* dprobe, float* dpos, double* error_out);
 /* (mset_grad, mset_waves, mset_simulate and the mset_recon_* tilt series use the multislice operator.) */
+
 
 /* The incident wave and the wave after each plane for position k at the resident state, i.e.
  * tmp_probe_wave and save_wave_f of STEP02_FORWARD_4D.m:27-35: interleaved complex [L + 1][ny][nx]. */

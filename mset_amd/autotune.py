@@ -3,8 +3,10 @@ workload itself.
 
 Tunables (both leave the kernels' arithmetic unchanged):
   slots  the persistent grid of the batched kernels (mset_set_slot_limit).  Fewer resident positions
-         hold less scratch (saved waves, gradient slabs and, at 256 x 256, the plane buffers) and keep
-         more of it in L2 / MALL; more of them hide memory latency better.
+         hold less scratch (saved waves and, at 256 x 256, the plane buffers) and keep more of it in L2 /
+         MALL; more of them hide memory latency better.  Gradient slabs are assigned by position, not by
+         workgroup, and a capped grid divides the slab count, so the sums are bitwise the same for every
+         candidate: tuning never changes a result.
   batch  positions per launch of an update pass (the mini-batch size), from the range the caller
          allows: it sets how many launches, reductions and updates a pass makes and how well the last
          wave of workgroups fills the device.
@@ -25,8 +27,9 @@ from . import lib
 _CACHE = {}
 
 
-def slot_candidates(n, fracs=(1.0, 0.75, 0.5), device=0):
-    """Grid sizes to try: fractions of the resident workgroups, rounded to whole CUs' worth."""
+def slot_candidates(n, fracs=(1.0, 0.5, 0.25), device=0):
+    """Grid sizes to try: fractions of the resident workgroups that divide their count (a capped grid
+    must divide the slab count, which is the resident workgroups')."""
     full = lib.batch_slots(n, device)
     out = []
     for f in fracs:

@@ -111,6 +111,7 @@ struct mset_ctx {
     DevBuf<float> V, sqrtM, x, y, dpos, Mtmp;
     DevBuf<float2> probe, prop, tw;
     DevBuf<double> err;
+    DevBuf<float2> mt_trans, mt_probeF;   // this tilt's transmission planes and fftn(probe) in a multi-tilt launch
     std::shared_ptr<Workspace> ws;
     int slots = 0;
     bool* profiling = nullptr;
@@ -237,14 +238,34 @@ PosArgs base_args(mset_ctx* c) {
     return A;
 }
 
-// workgroups that can be resident at once for the batched kernels (1 per CU at N = 128)
-int batch_slots(mset_ctx* c) {
+// workgroups of the batched kernels the device keeps resident at once (1 per CU at N = 128)
+int device_slots(mset_ctx* c) {
     int per_cu = 1;
     if (occupancy_batch(c->n, &per_cu) != hipSuccess || per_cu < 1) per_cu = 1;
-    int s = c->cus * per_cu;
+    return c->cus * per_cu;
+}
+
+// the persistent grid of the batched kernels: the resident workgroups, capped by the slot limit and the
+// positions
+int batch_slots(mset_ctx* c) {
+    int s = device_slots(c);
     if (c->ws->slot_limit > 0 && s > c->ws->slot_limit) s = c->ws->slot_limit;
     if (s > c->npos) s = c->npos;
     return s < 1 ? 1 : s;
+}
+
+// Gradient slabs of the context's batched launches: position p of a launch adds to slab p % S.  S depends
+// on the device and the positions only, never on the slot limit, so capping the grid leaves every sum
+// (and so every result) bitwise unchanged.
+int grad_slabs(mset_ctx* c) { return std::max(1, std::min(device_slots(c), c->npos)); }
+
+// grid of a batched launch of `count` positions on S slabs: at most `want` workgroups and, when slabs take
+// several positions (count > S), a divisor of S, so that each slab's positions run on one workgroup in order
+int launch_grid(int want, int count, int S) {
+    int g = std::max(1, std::min(want, count));
+    if (count > S)
+        while (S % g) --g;
+    return g;
 }
 
 int ensure_fixed_buffers(mset_ctx* c) {
@@ -285,8 +306,9 @@ int ensure_batch_buffers(mset_ctx* c) {
     CHK(W.save.ensure((size_t)s * save_planes * plane(c), "save waves"));
     W.recompute = rc;
     W.slot_bytes = plane(c) * (save_planes * sizeof(float2) + (size_t)planes(c) * sizeof(float));
-    CHK(W.gslab.ensure((size_t)s * c->L * plane(c), "gradient slabs"));
-    CHK(W.pslab.ensure((size_t)s * plane(c), "probe slabs"));
+    const int S = grad_slabs(c);
+    CHK(W.gslab.ensure((size_t)S * c->L * plane(c), "gradient slabs"));
+    CHK(W.pslab.ensure((size_t)S * plane(c), "probe slabs"));
     CHK(W.G.ensure((size_t)c->L * plane(c), "gradient"));
     CHK(W.PG.ensure(plane(c), "probe gradient"));
     CHK(c->dpos.ensure((size_t)2 * c->npos, "position terms"));
@@ -356,20 +378,23 @@ int issue_grad(mset_ctx* c, int first, int count, bool apply, int want) {
     CHK(ensure_batch_buffers(c));
     CHK(prepare_fixed(c));
     Workspace& W = *c->ws;
-    const int grid = std::min(c->slots, count);
+    const int S = grad_slabs(c);
+    const int grid = launch_grid(c->slots, count, S);
+    const int used = std::min(count, S);                 // slabs written by this launch
     PosArgs A = base_args(c);
     A.first = first;
     A.count = count;
     A.want = want;
-    // every slot of the grid processes at least one position, and its first one writes the slot's slabs
+    A.nslab = S;
+    // every used slab receives at least one position, and its first one writes it
     {
         Timed t(c, "position_batch");
         HIPCHK(launch_batch(c->n, A, grid, c->stream));
     }
-    HIPCHK(launch_reduce_slabs(W.gslab.p, grid, (size_t)planes(c) * plane(c), W.G.p, c->stream));
+    HIPCHK(launch_reduce_slabs(W.gslab.p, used, (size_t)planes(c) * plane(c), W.G.p, c->stream));
     if (want & WANT_PROBE) {
         // in place into slab 0 (each element is read and written by the same thread), then PG = ifftn
-        HIPCHK(launch_reduce_cslabs(W.pslab.p, grid, plane(c), W.pslab.p, c->stream));
+        HIPCHK(launch_reduce_cslabs(W.pslab.p, used, plane(c), W.pslab.p, c->stream));
         HIPCHK(launch_plane_fft(c->n, W.pslab.p, W.PG.p, c->tw.p, 1, 1.f, 0.f, W.planes.p, c->stream));
     }
     if (apply) {
@@ -488,7 +513,7 @@ int upload_impl(mset_ctx* c, const mset_params* p, const float* RVol, const floa
     HIPCHK(hipMemcpyAsync(c->x.p, xlist, sizeof(float) * c->npos, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->y.p, ylist, sizeof(float) * c->npos, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->Mtmp.p, measured, sizeof(float) * n2 * c->npos, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(launch_ingest_patterns(c->Mtmp.p, c->sqrtM.p, n, c->npos, c->stream));
+    HIPCHK(launch_ingest_patterns(c->Mtmp.p, c->sqrtM.p, n, c->npos, c->p.diffraction_transpose != 0, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->Mtmp.release();
     c->uploaded = true;
@@ -527,6 +552,10 @@ struct mset_recon {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int graph_key[3] = {-1, -1, -1};
+    // the error pass over every tilt in one launch: per-tilt table, work items, per-position errors
+    DevBuf<TiltArgs> tilt_tab;
+    DevBuf<int> items;
+    DevBuf<double> err_all;
 };
 
 namespace {
@@ -694,6 +723,7 @@ int mset_waves(mset_ctx* c, int k, float* waves_out) {
     A.count = 1;
     A.want = 0;
     A.recompute = 0;
+    A.nslab = 1;
     HIPCHK(launch_batch(c->n, A, 1, c->stream));
     HIPCHK(hipMemcpyAsync(waves_out, c->ws->save.p, sizeof(float2) * (size_t)(c->L + 1) * plane(c),
                           hipMemcpyDeviceToHost, c->stream));
@@ -965,17 +995,66 @@ int mset_recon_iterate(mset_recon* r, int path, int batch, int use_graph, int po
     return MSET_OK;
 }
 
+// The error pass of Run_MSET_recon.m:189-239 (run_error_mset, mset.cu:557-696) over every tilt: each tilt's
+// rotated volume, transmission planes and probe spectrum are prepared, then the positions of all tilts are
+// one persistent launch (no per-tilt launch tails or host synchronisations), and the per-position errors
+// come back in one copy.
 int mset_recon_error_pass(mset_recon* r, double* errs) {
     CHK(check_recon(r));
     if (!errs) return fail(MSET_EINVAL, "null output");
+    const int T = (int)r->tilts.size();
+    if (T >= (1 << (31 - ITEM_TILT_SHIFT))) return fail(MSET_EINVAL, "too many tilts for one error-pass launch");
     const size_t nv = (size_t)r->n * r->n * r->nz;
-    for (size_t j = 0; j < r->tilts.size(); ++j) {
+    mset_ctx* c0 = r->tilts[0];
+    std::vector<size_t> off(T + 1, 0);
+    for (int j = 0; j < T; ++j) {
+        if (r->tilts[j]->npos > ITEM_POS_MASK) return fail(MSET_EINVAL, "too many positions in one tilt");
+        off[j + 1] = off[j] + r->tilts[j]->npos;
+    }
+    const size_t tot = off[T];
+    CHK(r->err_all.ensure(tot, "error-pass errors"));
+    CHK(r->items.ensure(tot, "error-pass work items"));
+    CHK(r->tilt_tab.ensure(T, "error-pass tilt table"));
+    int grid = std::max(1, device_slots(c0));
+    if (r->ws->slot_limit > 0) grid = std::min(grid, r->ws->slot_limit);
+    grid = (int)std::min<size_t>((size_t)grid, tot);
+    if (plane_in_global(r->n)) {
+        const size_t need = (size_t)grid * 2 * plane(c0);
+        if (r->ws->planes.n < need) drop_graph(r);       // a captured update pass holds the current buffer
+        CHK(r->ws->planes.ensure(need, "plane buffers"));
+    }
+    std::vector<TiltArgs> tab(T);
+    std::vector<int> items(tot);
+    for (int j = 0; j < T; ++j) {
         mset_ctx* c = r->tilts[j];
+        CHK(set_method(c, MSET_METHOD_MSET));
         HIPCHK(hipMemcpyAsync(c->V.p, r->rec.p, sizeof(float) * nv, hipMemcpyDeviceToDevice, r->stream));
         CHK(rotate_volume(c, c->V.p, r->vec, r->vec + 3, r->vec + 6, r->angles[j].data(), true));
-        CHK(issue_fwd(c, nullptr));
-        CHK(read_errors(c, 0, c->npos, &errs[j]));
+        CHK(c->mt_trans.ensure((size_t)c->L * plane(c), "tilt transmission"));
+        CHK(c->mt_probeF.ensure(plane(c), "tilt probe spectrum"));
+        HIPCHK(launch_transmission(c->V.p, c->mt_trans.p, c->n, c->nz, binning(c), 0, c->p.sigma,
+                                   c->p.potential_pixelsize, r->stream));
+        HIPCHK(launch_plane_fft(c->n, c->probe.p, c->mt_probeF.p, c->tw.p, 0, 1.f, 0.f, r->ws->planes.p, r->stream));
+        tab[j] = TiltArgs{c->mt_trans.p, c->mt_probeF.p, c->sqrtM.p, c->x.p, c->y.p, r->err_all.p + off[j]};
+        for (int k = 0; k < c->npos; ++k) items[off[j] + k] = (j << ITEM_TILT_SHIFT) | k;   // tilt-major
     }
+    HIPCHK(hipMemcpyAsync(r->tilt_tab.p, tab.data(), sizeof(TiltArgs) * T, hipMemcpyHostToDevice, r->stream));
+    HIPCHK(hipMemcpyAsync(r->items.p, items.data(), sizeof(int) * tot, hipMemcpyHostToDevice, r->stream));
+    PosArgs A = base_args(c0);
+    A.tilts = r->tilt_tab.p;
+    A.order = r->items.p;
+    A.first = 0;
+    A.count = (int)tot;
+    A.planes = plane_in_global(r->n) ? r->ws->planes.p : nullptr;
+    {
+        Timed t(c0, "position_fwd");
+        HIPCHK(launch_fwd_tilts(r->n, A, grid, r->stream));
+    }
+    std::vector<double> e(tot);
+    HIPCHK(hipMemcpyAsync(e.data(), r->err_all.p, sizeof(double) * tot, hipMemcpyDeviceToHost, r->stream));
+    HIPCHK(hipStreamSynchronize(r->stream));
+    for (int j = 0; j < T; ++j)
+        errs[j] = host_error_sum(r->tilts[j], std::vector<double>(e.begin() + off[j], e.begin() + off[j + 1]));
     return MSET_OK;
 }
 

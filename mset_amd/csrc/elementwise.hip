@@ -16,14 +16,15 @@ static inline dim3 grid_for(size_t n, int per_thread = 1) {
 
 // sqrt(ifftshift(M)) of every pattern (src/CUDA/mset.cu:206-210 does this per position per visit;
 // here it is done once at ingest), stored transposed for the row pass that reads it:
-// out[k][x][y] = sqrt(M[k][(y + N/2) % N][(x + N/2) % N])
-__global__ void k_ingest(const float* __restrict__ M, float* __restrict__ out, int n, long long npos) {
+// out[k][x][y] = sqrt(M[k][(y + N/2) % N][(x + N/2) % N]).  transpose: each pattern is first transposed
+// (STEM_data.diffraction_transpose, STEP00_INIT.m:58-73), i.e. M[k][x][y] is read instead.
+__global__ void k_ingest(const float* __restrict__ M, float* __restrict__ out, int n, long long npos, int transpose) {
     const long long n2 = (long long)n * n, tot = n2 * npos;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < tot; i += (long long)gridDim.x * blockDim.x) {
         const long long k = i / n2;
         const int r = (int)(i - k * n2), x = r / n, y = r % n;
         const int ys = (y + n / 2) % n, xs = (x + n / 2) % n;
-        out[i] = sqrtf(M[k * n2 + (long long)ys * n + xs]);
+        out[i] = sqrtf(M[k * n2 + (transpose ? (long long)xs * n + ys : (long long)ys * n + xs)]);
     }
 }
 
@@ -106,8 +107,8 @@ __global__ void k_positivity(float* V, size_t n) {
         V[i] = V[i] < 0.f ? 0.f : V[i];
 }
 
-hipError_t launch_ingest_patterns(const float* M, float* sqrtM, int n, long long npos, hipStream_t st) {
-    hipLaunchKernelGGL(k_ingest, grid_for((size_t)n * n * npos), dim3(256), 0, st, M, sqrtM, n, npos);
+hipError_t launch_ingest_patterns(const float* M, float* sqrtM, int n, long long npos, int transpose, hipStream_t st) {
+    hipLaunchKernelGGL(k_ingest, grid_for((size_t)n * n * npos), dim3(256), 0, st, M, sqrtM, n, npos, transpose);
     return hipGetLastError();
 }
 

@@ -7,6 +7,19 @@ namespace mset {
 enum { GRAD_REF = 0, GRAD_EXACT = 1 };
 enum { WANT_PROBE = 1, WANT_POS = 2 };
 
+// Per-tilt inputs and outputs of a launch that runs the positions of several tilts (a table in device
+// memory; position k of tilt j is the work item (j << 24) | k).
+struct TiltArgs {
+    const float2* trans;         // transmission planes of the tilt's rotated volume
+    const float2* probeF;        // fftn(the tilt's probe)
+    const float* sqrtM;          // the tilt's patterns (sqrt, ifftshifted, transposed)
+    float* xlist;                // the tilt's scan positions
+    float* ylist;
+    double* err_pos;             // per-position errors of the tilt
+};
+constexpr int ITEM_TILT_SHIFT = 24;                  // work item = (tilt << 24) | position
+constexpr int ITEM_POS_MASK = (1 << ITEM_TILT_SHIFT) - 1;
+
 // Arguments of the per-position kernels (passed by value).  Plane data is [y][x], N x N.
 struct PosArgs {
     int nz, L, bin;              // volume depth, number of planes ceil(nz/bin), slice binning
@@ -26,13 +39,16 @@ struct PosArgs {
     const float2* probeF;        // K_BATCH / K_FWD: fftn(probe)
     float* xlist;                // scan positions (MATLAB 1-based pixels); K_PLAIN updates them
     float* ylist;
-    const int* order;            // optional list of position indices
+    const int* order;            // optional list of position indices (work items when tilts is set)
+    const TiltArgs* tilts;       // K_FWD over several tilts: per-tilt pointers, indexed by the work item
     int first, count;            // K_BATCH / K_FWD: positions order[first .. first+count)
+    int obase;                   // err_pos and dpos are indexed k - obase (0: by position index)
     double* err_pos;             // per position: sum over pixels of (sqrt M - sqrt I)^2
     float* inten;                // K_FWD, optional: |Psi|^2 per position, DC-centred (MATLAB layout)
     float2* save;                // per-slot scratch: (L + 2) planes, or RECOMPUTE_PLANES when recomputing
-    float* gslab;                // per-slot object-gradient slab, L planes (K_PLAIN: slot 0, this position)
-    float2* pslab;               // K_BATCH: per-slot probe-gradient plane (Fourier domain)
+    float* gslab;                // object-gradient slabs, L planes each (K_PLAIN: slab 0, this position)
+    float2* pslab;               // K_BATCH: probe-gradient slabs, one plane each (Fourier domain)
+    int nslab;                   // K_BATCH: position p of the launch adds to slab p % nslab
     float2* planes;              // N = 256: per-slot plane buffers (2 N^2), the tiled engine's state
     float* dpos;                 // K_BATCH: per-position update terms (2 per position)
 };
@@ -45,13 +61,15 @@ int position_threads(int n);
 hipError_t launch_plain(int n, const PosArgs& A, int k, hipStream_t st);
 hipError_t launch_batch(int n, const PosArgs& A, int grid, hipStream_t st);
 hipError_t launch_fwd(int n, const PosArgs& A, int grid, hipStream_t st);
+// forward + loss over the work items A.order[0 .. A.count) of several tilts (A.tilts)
+hipError_t launch_fwd_tilts(int n, const PosArgs& A, int grid, hipStream_t st);
 hipError_t launch_plane_fft(int n, const float2* in, float2* out, const float2* tw, int dir, float alpha, float beta,
                             float2* planes, hipStream_t st);
 bool plane_in_global(int n);     // the position kernels keep the plane in global memory (planes buffer)
 hipError_t occupancy_batch(int n, int* blocks_per_cu);
 
 // element-wise kernels (elementwise.hip)
-hipError_t launch_ingest_patterns(const float* M, float* sqrtM, int n, long long npos, hipStream_t st);
+hipError_t launch_ingest_patterns(const float* M, float* sqrtM, int n, long long npos, int transpose, hipStream_t st);
 hipError_t launch_transmission(const float* V, float2* trans, int n, int nz, int bin, int sset, float sigma, float dx,
                                hipStream_t st);
 hipError_t launch_fill(void* p, size_t bytes, hipStream_t st);

@@ -74,11 +74,12 @@ struct AdjOp {
 };
 
 
-// first: the slot's first position of the launch, which writes the slot's slabs instead of adding to
-// them (so the slabs need no clearing launch)
+// slot: the workgroup's scratch (waves, plane buffers); slab: the gradient slab the position adds to;
+// first: the slab's first position of the launch, which writes the slab instead of adding to it (so the
+// slabs need no clearing launch)
 template <int N, int NT, int KIND, bool RC = false, class Eng>
-__device__ __forceinline__ void process_position(const PosArgs& A, int k, int slot, bool first, Eng& E, float2* phx,
-                                                 float2* phy, double* red) {
+__device__ __forceinline__ void process_position(const PosArgs& A, int k, int slot, int slab, bool first, Eng& E,
+                                                 float2* phx, float2* phy, double* red) {
     constexpr int N2 = N * N;
     const int t = threadIdx.x;
     const int L = A.L;
@@ -220,7 +221,7 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
         }
     }
     err = block_sum<NT>(err, red);
-    if (t == 0) A.err_pos[k] = err;
+    if (t == 0) A.err_pos[k - A.obase] = err;
     if constexpr (KIND == K_FWD) return;
 
     // ---- adjoint sweep (STEP03_BACK.m:12-24, mset.cu:232-264)
@@ -268,8 +269,8 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
             }
         }
         const BufC ws(wsp, N2);
-        const BufF g(A.gslab + ((size_t)slot * L + a) * N2, N2);
-        const BufF gin(A.gslab + ((size_t)slot * L + a) * N2, first ? 0u : N2);   // empty view: reads 0
+        const BufF g(A.gslab + ((size_t)slab * L + a) * N2, N2);
+        const BufF gin(A.gslab + ((size_t)slab * L + a) * N2, first ? 0u : N2);   // empty view: reads 0
         auto adjoint = [&](float2 v, float2 t, float2 w, float& gr) {
             const float2 b = cscale(v, inv_n2);
             const float2 tc = cconj(t);
@@ -327,8 +328,8 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
     // ---- probe term in the Fourier domain: shift(bw, -d) = ifftn(fftn(bw) .* conj(Phi)); slab [x][y]
     if constexpr (KIND == K_BATCH) {
         if (want_probe) {
-            const BufC ps(A.pslab + (size_t)slot * N2, N2);
-            const BufC psin(A.pslab + (size_t)slot * N2, first ? 0u : N2);   // empty view: reads 0
+            const BufC ps(A.pslab + (size_t)slab * N2, N2);
+            const BufC psin(A.pslab + (size_t)slab * N2, first ? 0u : N2);   // empty view: reads 0
             E.each_R(staged<16>([&](const Pt& p) { return psin[p.e]; },
                                 [&](const Pt& p, float2& v, float2 o) { ps.put(p.e, cadd(o, cmulc(v, phase(p.y, p.x)))); }));
         }
@@ -372,8 +373,8 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
 
     if constexpr (KIND == K_BATCH) {
         if (t == 0 && want_pos) {
-            A.dpos[2 * k] = (float)dr;
-            A.dpos[2 * k + 1] = (float)dc;
+            A.dpos[2 * (k - A.obase)] = (float)dr;
+            A.dpos[2 * (k - A.obase) + 1] = (float)dc;
         }
     } else {
         // ---- sequential updates (STEP03_BACK.m:84-92, Run_MSET_recon.m:155-157)
@@ -421,29 +422,48 @@ template <int N>
 __global__ __launch_bounds__(Cfg<N>::NT, Cfg<N>::MINB) void k_plain(PosArgs A, int k) {
     constexpr int NT = Cfg<N>::NT;
     MSET_LDS(N, NT)
-    process_position<N, NT, K_PLAIN>(A, k, 0, true, E, phx, phy, red);
+    process_position<N, NT, K_PLAIN>(A, k, 0, 0, true, E, phx, phy, red);
 }
 
 // RC: the recompute variant (A.recompute), a separate instantiation so that the store variant carries
-// none of its code
+// none of its code.  Position p of the launch adds to slab p % nslab.  The host launches either at most
+// nslab positions or a grid that divides nslab, so every slab's positions run on one workgroup in
+// increasing p: each slab sums the same positions in the same order whatever the grid size.
 template <int N, bool RC>
 __global__ __launch_bounds__(Cfg<N>::NT, Cfg<N>::MINB) void k_batch(PosArgs A) {
     constexpr int NT = Cfg<N>::NT;
     MSET_LDS(N, NT)
     for (int p = blockIdx.x; p < A.count; p += gridDim.x) {
         const int k = A.order ? A.order[A.first + p] : A.first + p;
-        process_position<N, NT, K_BATCH, RC>(A, k, blockIdx.x, p == (int)blockIdx.x, E, phx, phy, red);
+        const int slab = p % A.nslab;
+        process_position<N, NT, K_BATCH, RC>(A, k, blockIdx.x, slab, p < A.nslab, E, phx, phy, red);
         __syncthreads();
     }
 }
 
-template <int N>
+// MT: the work items are positions of several tilts (A.tilts, A.order), so a whole error pass over a tilt
+// series is one persistent launch; a separate instantiation, so the one-tilt kernel carries none of it
+template <int N, bool MT>
 __global__ __launch_bounds__(Cfg<N>::NT, Cfg<N>::MINB) void k_fwd(PosArgs A) {
     constexpr int NT = Cfg<N>::NT;
     MSET_LDS(N, NT)
     for (int p = blockIdx.x; p < A.count; p += gridDim.x) {
-        const int k = A.order ? A.order[A.first + p] : A.first + p;
-        process_position<N, NT, K_FWD>(A, k, blockIdx.x, p == (int)blockIdx.x, E, phx, phy, red);
+        if constexpr (MT) {
+            const int item = A.order[p];
+            const TiltArgs& T = A.tilts[item >> ITEM_TILT_SHIFT];
+            PosArgs B = A;
+            B.trans = const_cast<float2*>(T.trans);
+            B.probeF = T.probeF;
+            B.sqrtM = T.sqrtM;
+            B.xlist = T.xlist;
+            B.ylist = T.ylist;
+            B.err_pos = T.err_pos;
+            B.obase = 0;
+            process_position<N, NT, K_FWD>(B, item & ITEM_POS_MASK, blockIdx.x, blockIdx.x, true, E, phx, phy, red);
+        } else {
+            const int k = A.order ? A.order[A.first + p] : A.first + p;
+            process_position<N, NT, K_FWD>(A, k, blockIdx.x, blockIdx.x, p == (int)blockIdx.x, E, phx, phy, red);
+        }
         __syncthreads();
     }
 }
@@ -527,7 +547,12 @@ hipError_t launch_batch(int n, const PosArgs& A, int grid, hipStream_t st) {
 }
 
 hipError_t launch_fwd(int n, const PosArgs& A, int grid, hipStream_t st) {
-    MSET_DISPATCH(n, hipLaunchKernelGGL(k_fwd<NN>, dim3(grid), dim3(Cfg<NN>::NT), 0, st, A));
+    MSET_DISPATCH(n, hipLaunchKernelGGL((k_fwd<NN, false>), dim3(grid), dim3(Cfg<NN>::NT), 0, st, A));
+    return hipGetLastError();
+}
+
+hipError_t launch_fwd_tilts(int n, const PosArgs& A, int grid, hipStream_t st) {
+    MSET_DISPATCH(n, hipLaunchKernelGGL((k_fwd<NN, true>), dim3(grid), dim3(Cfg<NN>::NT), 0, st, A));
     return hipGetLastError();
 }
 

@@ -18,7 +18,8 @@ import numpy as np
 from . import lib, synth
 
 
-def run(positions=256, repeat=2, seed=4):
+def make_workload(positions=256, seed=4):
+    """The 256^3 workload with a square scan of about `positions` positions and its simulated data."""
     side = int(round(np.sqrt(positions)))
     synth.CONFIGS['C4-256'] = dict(vol=256, P=256, dx=0.4, kV=300.0, alpha=21.0, defocus=0.0, scan=(side, side),
                                    step_A=0.8, jitter=0.5, tilts=[0.0], slice_A=0.4, dose=1e5, occupancy=0.08)
@@ -26,6 +27,12 @@ def run(positions=256, repeat=2, seed=4):
     ctx = lib.Context(0)
     wl.simulate(ctx)
     ctx.close()
+    return wl
+
+
+def run(positions=256, repeat=2, seed=4, wl=None):
+    if wl is None:
+        wl = make_workload(positions, seed)
     xl, yl, M = wl.xlists[0], wl.ylists[0], wl.measured[0]
     V0 = 0.5 * wl.truth
     out = dict(workload='256^3 volume at 0.4 A, probe 256^2, %d planes, %d positions, 1 tilt' % (wl.L, len(xl)),

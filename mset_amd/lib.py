@@ -43,7 +43,7 @@ class Params(C.Structure):
                 ('n_scan_x', C.c_int32), ('n_scan_y', C.c_int32), ('num_scan_pos', C.c_int32),
                 ('slice_binning', C.c_int32),
                 ('alpha', C.c_float), ('sigma', C.c_float), ('E0', C.c_float), ('potential_pixelsize', C.c_float),
-                ('step_size', C.c_float * 3), ('grad_mode', C.c_int32)]
+                ('step_size', C.c_float * 3), ('grad_mode', C.c_int32), ('diffraction_transpose', C.c_int32)]
 
 
 _LIB = None
@@ -187,7 +187,8 @@ class Context:
 
     # ---------------------------------------------------------------- MEX-compatible
     def upload(self, RVol, probe, prop2D, xlist, ylist, measured, sigma, dx, slice_binning=1,
-               step_size=(0.0, 0.0, 0.0), n_scan=None, grad_mode=GRAD_REF, alpha=0.0, E0=0.0):
+               step_size=(0.0, 0.0, 0.0), n_scan=None, grad_mode=GRAD_REF, alpha=0.0, E0=0.0,
+               diffraction_transpose=False):
         RVol = _f32(RVol)
         nz, ny, nx = RVol.shape
         xl, yl = _f32(xlist).ravel(), _f32(ylist).ravel()
@@ -200,7 +201,7 @@ class Context:
         p = Params(nx=nx, ny=ny, nz=nz, n_scan_x=int(n_scan[0]), n_scan_y=int(n_scan[1]), num_scan_pos=npos,
                    slice_binning=int(slice_binning), alpha=float(alpha), sigma=float(sigma), E0=float(E0),
                    potential_pixelsize=float(dx), step_size=(C.c_float * 3)(*[float(s) for s in step_size]),
-                   grad_mode=int(grad_mode))
+                   grad_mode=int(grad_mode), diffraction_transpose=int(bool(diffraction_transpose)))
         pr, pp = _c64(probe), _c64(prop2D)
         _chk(self._L.mset_upload(self._h, C.byref(p), _ptr(RVol), _ptr(pr), _ptr(pp), _ptr(xl), _ptr(yl), _ptr(M)))
         self.params = p
@@ -338,13 +339,14 @@ class Recon:
     """A device-resident tilt series: Run_MSET_recon's update and error passes on the GPU."""
 
     def __init__(self, rec, prop2D, n_tilts, sigma, dx, slice_binning=1, step_size=(0.0, 0.0, 0.0),
-                 grad_mode=GRAD_REF, vecs=((0, 0, 1), (0, 1, 0), (1, 0, 0)), device=0):
+                 grad_mode=GRAD_REF, vecs=((0, 0, 1), (0, 1, 0), (1, 0, 0)), device=0, diffraction_transpose=False):
         self._L = load()
         rec = _f32(rec)
         nz, ny, nx = rec.shape
         p = Params(nx=nx, ny=ny, nz=nz, n_scan_x=1, n_scan_y=1, num_scan_pos=1, slice_binning=int(slice_binning),
                    sigma=float(sigma), potential_pixelsize=float(dx),
-                   step_size=(C.c_float * 3)(*[float(s) for s in step_size]), grad_mode=int(grad_mode))
+                   step_size=(C.c_float * 3)(*[float(s) for s in step_size]), grad_mode=int(grad_mode),
+                   diffraction_transpose=int(bool(diffraction_transpose)))
         v = np.ascontiguousarray(vecs, np.float64).ravel()
         h = C.c_void_p()
         _chk(self._L.mset_recon_create(C.byref(h), int(device), C.byref(p), int(n_tilts), _ptr(v), _ptr(rec),

@@ -9,8 +9,12 @@ one job (an update pass over all its tilts, followed by its loss).  Requests are
     copied to the host and restored exactly on re-admission.  The earliest request that waits for
     memory holds back the resident jobs it needs evicted (no new requests start on them), so a large
     job is not starved by a stream of requests for small resident ones;
-  * up to `workers` requests run at once, each job on its own HIP stream, so work of different jobs
-    overlaps on the device; one job never runs two requests at a time, so its iterations stay ordered;
+  * up to `workers` requests run at once, each job on its own HIP stream.  A request starting while k
+    others run gets 1/2^ceil(log2(k+1)) of the device's resident workgroups as its persistent grid (all
+    of them when it runs alone), so the persistent kernels of concurrent jobs co-reside on the CUs and
+    their work overlaps instead of queueing behind one another.  Gradient slabs are assigned by position,
+    not by workgroup, so the grid share never changes a result.  One job never runs two requests at a
+    time, so its iterations stay ordered;
   * every request's latency (arrival to completion) is recorded; p50 / p99 and the aggregate
     probe-position-slice throughput are reported.
 
@@ -126,6 +130,8 @@ class Request:
     seq: int
     start: float = 0.0
     end: float = 0.0
+    slots: int = 0                # the persistent grid this request ran with
+    concurrent: int = 0           # requests running (this one included) when it started
     error: float = float('nan')
     admitted: bool = False        # this request made its job resident
     failure: str = ''             # set when the request failed (the job's state is unchanged)
@@ -154,7 +160,8 @@ class Server:
             from . import lib
             budget_bytes = int(0.9 * lib.device_memory(device)[0])
         self.budget = int(budget_bytes)
-        self.jobs = {s.job_id: _Job(s, footprint_bytes(s, slots_of(s.P))) for s in specs}
+        self.slots_of = {s.P: slots_of(s.P) for s in specs}
+        self.jobs = {s.job_id: _Job(s, footprint_bytes(s, self.slots_of[s.P])) for s in specs}
         for j in self.jobs.values():
             if j.bytes > self.budget:
                 raise ValueError('job %d needs ~%.1f GB, more than the budget %.1f GB'
@@ -166,6 +173,9 @@ class Server:
         self.pending = []
         self.completed = []
         self.resident_bytes = 0
+        self.running = 0
+        self.workers = int(workers)
+        self.busy_log = []            # (time, requests running) at every start and end
         self.evictions = 0
         self.closing = False
         self.seq = 0
@@ -237,6 +247,16 @@ class Server:
                     need -= j.bytes
         return None, None
 
+    def _share(self, P, running):
+        """Persistent grid of a request that starts with `running` requests on the device (itself included):
+        the resident workgroups divided by the next power of two, so requests that start together fit on the
+        device side by side, and every share divides the slab count.  A request keeps the share it started
+        with for its whole pass."""
+        div = 1
+        while div < running:
+            div *= 2
+        return max(1, self.slots_of[P] // div)
+
     def _worker(self):
         while True:
             with self.cv:
@@ -256,16 +276,21 @@ class Server:
                     self.resident_bytes += job.bytes
                 for j in evict:
                     self.resident_bytes -= j.bytes
-            for j in evict:                                        # outside the lock: device work
-                j.saved = j.recon.get()
-                j.recon.close()
-                j.recon = None
+                self.running += 1
+                req.concurrent = self.running
+                req.slots = self._share(job.spec.P, self.running)
+                self.busy_log.append((self.clock(), self.running))
             req.start = self.clock()
             try:
+                for j in evict:                                    # outside the lock: device work
+                    j.saved = j.recon.get()
+                    j.recon.close()
+                    j.recon = None
                 if admit:
                     job.recon = self.factory(job.spec, job.saved)
                     job.saved = None
                     req.admitted = True
+                job.recon.set_slot_limit(req.slots)
                 req.error = job.recon.iterate(1, self.batch, self.use_graph, True, want_error=True)
             except Exception as e:                                  # report it on the request, keep serving
                 req.failure = '%s: %s' % (type(e).__name__, e)
@@ -279,7 +304,12 @@ class Server:
                     job.errors.append(req.error)
                 for j in evict:
                     j.busy = False
-                    self.evictions += 1
+                    if j.recon is None:
+                        self.evictions += 1
+                    else:                                          # its eviction failed: it is still resident
+                        self.resident_bytes += j.bytes
+                self.running -= 1
+                self.busy_log.append((req.end, self.running))
                 self.completed.append(req)
                 self.cv.notify_all()
             req.done.set()
@@ -290,11 +320,16 @@ class Server:
         t0 = min(r.arrival for r in self.completed)
         t1 = max(r.end for r in self.completed)
         units = sum(self.jobs[r.job_id].spec.pos_slices() for r in self.completed)
+        # time with two or more requests on the device, from the start / end log
+        log = sorted(self.busy_log)
+        overlap = sum(b[0] - a[0] for a, b in zip(log, log[1:]) if a[1] >= 2)
         return dict(requests=len(self.completed), p50_ms=float(np.percentile(lat, 50)),
                     p99_ms=float(np.percentile(lat, 99)), mean_ms=float(lat.mean()),
                     throughput=units / (t1 - t0), unit='probe-position·slices/s', makespan_s=t1 - t0,
                     admissions=sum(r.admitted for r in self.completed), evictions=self.evictions,
-                    budget_gb=self.budget / 1e9)
+                    budget_gb=self.budget / 1e9, overlap_s=overlap,
+                    max_concurrent=max((r.concurrent for r in self.completed), default=0),
+                    failures=sum(bool(r.failure) for r in self.completed))
 
 
 def poisson_arrivals(n_requests, rate, n_jobs, seed):

@@ -33,9 +33,10 @@ def test_library_exports_every_declared_symbol():
 
 def test_params_struct_layout():
     from mset_amd.lib import Params
-    # 7 int32 + 4 float + float[3] + int32 = 15 * 4 bytes, no padding
-    assert C.sizeof(Params) == 15 * 4
+    # 7 int32 + 4 float + float[3] + 2 int32 = 16 * 4 bytes, no padding
+    assert C.sizeof(Params) == 16 * 4
     assert Params.step_size.offset == 11 * 4 and Params.grad_mode.offset == 14 * 4
+    assert Params.diffraction_transpose.offset == 15 * 4
 
 
 def test_null_context_is_an_error_not_a_crash():

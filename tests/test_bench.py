@@ -27,7 +27,8 @@ def test_more_than_one_gpu_is_refused_with_a_message():
 
 @pytest.mark.gpu
 def test_bench_prints_one_json_line_with_roofline_and_cpu_baseline():
-    r = _bench('--steps', '1', '--warmup', '0', '--no-autotune', '--plain-tilts', '0', '--cpu-positions', '4')
+    r = _bench('--workload', 'C2', '--steps', '1', '--warmup', '0', '--no-autotune', '--plain-tilts', '0',
+               '--cpu-positions', '4', '--object-only-steps', '1')
     assert r.returncode == 0, r.stderr[-2000:]
     lines = [l for l in r.stdout.splitlines() if l.strip()]
     assert len(lines) == 1
@@ -41,4 +42,8 @@ def test_bench_prints_one_json_line_with_roofline_and_cpu_baseline():
     assert rf['bound'] == 'hbm' and rf['peak'] == 8000.0 and 0 < rf['frac'] < 1
     assert abs(rf['achieved'] / rf['peak'] - rf['frac']) < 1e-9
     cb = d['cpu_baseline']
-    assert cb['kind'] == 'port' and cb['cores'] >= 1 and cb['value'] > 0
+    assert cb['kind'] == 'port' and cb['cores'] >= 1 and cb['value'] > 0 and cb['nproc'] >= cb['cores']
+    assert cb['single_thread']['cores'] == 1 and cb['single_thread']['value'] > 0
+    # all three gradient terms are formed in the timed pass; the object-only pass is reported beside it
+    assert all(s != 0 for s in d['config']['step_size'])
+    assert d['object_only']['step_size'][1:] == [0.0, 0.0] and d['object_only']['value'] > 0

@@ -50,6 +50,77 @@ def test_c1_update_pass_plain_vs_oracle():
     assert rel(rec - rec0, orec - rec0) < 1e-3
 
 
+def test_c2_tilt_batched_with_slot_reuse_vs_oracle():
+    """One C2 tilt (128 x 128, 16 planes) at a state away from the data (half the true potential): the batched
+    gradient over 32 of its positions on 8 slots, four positions per slot, against the FP64 oracle.  An FP32
+    chain of single-precision FFTs puts the position terms within ~1.5e-4 of FP64 at this state."""
+    from mset_amd import lib
+    ctx = lib.Context(0)
+    wl = _workload('C2', ctx, tilts=[2])
+    k = np.arange(0, wl.npos_of(0), 32)
+    V = (0.5 * wl.truth).astype(np.float64)
+    xl, yl, M = wl.xlists[0][k], wl.ylists[0][k], wl.measured[0][k]
+    ctx.upload(V, wl.probe, wl.prop, xl, yl, M, wl.sigma, wl.dx, slice_binning=wl.binning, n_scan=(len(k), 1))
+    ctx.set_slot_limit(8)
+    dV, dp, dpos, err = ctx.grad()
+    w = ctx.waves(5)
+    ctx.close()
+    odV, odp, odpos, oep = O.grad_fixed(V, wl.probe, wl.prop, np.conj(wl.prop), xl, yl, M, wl.binning, wl.sigma,
+                                        wl.dx)
+    psi0 = O.subpixel_shift(wl.probe, xl[5], yl[5])
+    save, _, _, _ = O.forward(V, psi0, wl.prop, M[5], wl.binning, wl.sigma, wl.dx)
+    r = dict(waves=max(rel(w[a + 1], save[a]) for a in range(wl.L)), loss=abs(err / (oep.sum() / len(k)) - 1),
+             dV=rel(dV, odV), dprobe=rel(dp, odp), dpos=rel(dpos, odpos))
+    print('C2 (32 positions on 8 slots) FP32 vs FP64 relative errors', r, flush=True)
+    assert r['waves'] < 1e-4 and r['loss'] < 1e-4
+    assert r['dV'] < 1e-3 and r['dprobe'] < 1e-3 and r['dpos'] < 1e-3
+
+
+def test_c2_full_size_properties():
+    """At the full C2 size (11 tilts x 1024 positions, 16 planes): the batched gradient of a tilt is bitwise
+    repeatable and additive over position ranges (per-position terms bitwise, sums to FP32 reordering), and
+    a graph-captured update pass of the 11-tilt series replays to the eager pass's state, bitwise, with
+    eager work and synchronisations between the replays."""
+    from mset_amd import lib
+    ctx = lib.Context(0)
+    wl = _workload('C2', ctx)
+    j = 4
+    V = (0.5 * wl.truth).astype(np.float32)
+    ctx.upload(V, wl.probe, wl.prop, wl.xlists[j], wl.ylists[j], wl.measured[j], wl.sigma, wl.dx,
+               slice_binning=wl.binning, n_scan=wl.n_scans[j])
+    a = ctx.grad()
+    b = ctx.grad()
+    for x, y in zip(a[:3], b[:3]):
+        assert np.array_equal(x, y)
+    assert a[3] == b[3]
+    lo, hi = ctx.grad(0, 400), ctx.grad(400, 624)
+    assert np.array_equal(np.concatenate([lo[2], hi[2]]), a[2])
+    assert rel(lo[0] + hi[0], a[0]) < 1e-5 and rel(lo[1] + hi[1], a[1]) < 1e-5
+    assert abs((lo[3] + hi[3]) / a[3] - 1) < 1e-12
+    ctx.close()
+
+    steps = (100.0, 0.02, 1e8)
+    r = lib.Recon(V, wl.prop, len(wl.angles), wl.sigma, wl.dx, slice_binning=wl.binning, step_size=steps)
+    try:
+        for i, ang in enumerate(wl.angles):
+            r.set_tilt(i, ang, wl.probe, wl.xlists[i], wl.ylists[i], wl.measured[i], n_scan=wl.n_scans[i])
+        r.snapshot()
+        r.iterate(lib.PATH_BATCHED, 1024, use_graph=False, want_error=False)
+        eager = r.get()
+        r.restore()
+        r.iterate(lib.PATH_BATCHED, 1024, use_graph=True, want_error=False)      # eager pass, then capture
+        for rep in range(3):
+            r.restore()
+            r.error_pass()                                                       # eager work between replays
+            r.iterate(lib.PATH_BATCHED, 1024, use_graph=True, want_error=False)  # replay
+            got = r.get()
+            assert np.array_equal(got[0], eager[0]) and np.array_equal(got[1], eager[1]), rep
+            for gx, ex in zip(got[2] + got[3], eager[2] + eager[3]):
+                assert np.array_equal(gx, ex), rep
+    finally:
+        r.close()
+
+
 def test_c3_fp32_gradient_vs_fp64_oracle():
     from mset_amd import lib
     ctx = lib.Context(0)
@@ -60,8 +131,10 @@ def test_c3_fp32_gradient_vs_fp64_oracle():
     ctx.upload(V, wl.probe, wl.prop, xl, yl, M, wl.sigma, wl.dx, slice_binning=wl.binning, n_scan=(len(k), 1))
     dV, dp, dpos, err = ctx.grad()
     w = ctx.waves(0)
+    one = [ctx.grad(i, 1) for i in range(len(k))]           # per-position terms
     ctx.close()
-    odV, odp, odpos, oep = O.grad_fixed(V, wl.probe, wl.prop, np.conj(wl.prop), xl, yl, M, wl.binning, wl.sigma, wl.dx)
+    bprop = np.conj(wl.prop)
+    odV, odp, odpos, oep = O.grad_fixed(V, wl.probe, wl.prop, bprop, xl, yl, M, wl.binning, wl.sigma, wl.dx)
     psi0 = O.subpixel_shift(wl.probe, xl[0], yl[0])
     save, _, _, _ = O.forward(V, psi0, wl.prop, M[0], wl.binning, wl.sigma, wl.dx)
     r = dict(exit_wave=rel(w[-1], save[-1]), loss=abs(err / (oep.sum() / len(k)) - 1), dV=rel(dV, odV),
@@ -69,17 +142,26 @@ def test_c3_fp32_gradient_vs_fp64_oracle():
     print('C3 FP32 vs FP64 relative errors', r, flush=True)
     assert r['exit_wave'] < 1e-4 and r['loss'] < 1e-4
     assert r['dV'] < 1e-3 and r['dprobe'] < 1e-3
-    # The position terms sum Re(grad(conj psi0) . bw) over 65536 pixels and cancel almost completely on this
-    # weak-phase state, so their relative error measures cancellation, not accuracy.  Bound the error by the
-    # summands' magnitude instead (MATLAB gradient() is numpy's edge_order=1 gradient).
+    # The position terms d = Re sum(g . bw), g = MATLAB gradient(conj psi0) (numpy's edge_order=1 gradient),
+    # cancel over the 65536 pixels by a factor of ~1e4 on this weak-phase, noisy state, so FP32 errors of the
+    # back wave bw are amplified in d: an independent FP32 chain (single-precision FFTs) shows 1.5e-3 to 8.5e-3,
+    # and forming the products in FP64 does not change it.  The bound used here follows from Cauchy-Schwarz:
+    # |d_dev - d| <= (eps_bw + eps_g + 2 eps_32) ||g|| ||bw||, where eps_bw is the device's own relative error of
+    # bw, read from its one-position probe term (a unitary shift of bw), and 1e-6 covers eps_g + 2 eps_32.
     for i in range(len(k)):
         p0 = O.subpixel_shift(wl.probe, xl[i], yl[i])
         sv, _, resi, _ = O.forward(V, p0, wl.prop, M[i], wl.binning, wl.sigma, wl.dx)
-        _, bw, _ = O.back(V, sv, p0, resi, np.conj(wl.prop), wl.binning, wl.sigma, wl.dx, xl[i], yl[i], O.MODE_REF)
+        _, bw, _ = O.back(V, sv, p0, resi, bprop, wl.binning, wl.sigma, wl.dx, xl[i], yl[i], O.MODE_REF)
+        _, odp1, odpos1, _ = O.grad_fixed(V, wl.probe, wl.prop, bprop, xl[i:i + 1], yl[i:i + 1], M[i:i + 1],
+                                          wl.binning, wl.sigma, wl.dx)
+        eps_bw = rel(one[i][1], odp1)
         gy, gx = np.gradient(np.conj(p0))
         for ax, gg in ((0, gx), (1, gy)):
-            scale = np.sum(np.abs(gg) * np.abs(bw))
-            assert abs(dpos[i, ax] - odpos[i, ax]) <= 1e-4 * scale, (i, ax, dpos[i, ax], odpos[i, ax], scale)
+            bound = (eps_bw + 1e-6) * np.linalg.norm(gg) * np.linalg.norm(bw)
+            e_all, e_one = abs(dpos[i, ax] - odpos[i, ax]), abs(one[i][2][0, ax] - odpos1[0, ax])
+            print('  position %d axis %d: rel %.2e, error / bound %.3f (eps_bw %.1e)'
+                  % (i, ax, e_all / abs(odpos[i, ax]), e_all / bound, eps_bw), flush=True)
+            assert e_all <= bound and e_one <= bound, (i, ax, e_all, e_one, bound)
 
 
 def test_c4_gradients_vs_central_differences():
@@ -155,3 +237,30 @@ def test_c4_gradients_vs_central_differences():
     odV, odp, odpos, _ = O.grad_fixed(V, wl.probe, wl.prop, np.conj(wl.prop), xl, yl, M, wl.binning, wl.sigma,
                                       wl.dx, mode=O.MODE_REF)
     assert rel(rdV, odV) < 1e-3 and rel(rdp, odp) < 1e-3 and rel(rdpos, odpos) < 1e-3
+
+
+def test_c4_store_vs_recompute_at_256_cubed():
+    """The memory case of the adjoint-verification configuration: a 256^3 volume with one-voxel slices (256
+    propagation planes), probe 256^2, 64 positions.  Recomputing the exit waves keeps 5 wave planes per
+    resident position instead of 258 and gives the stored-wave gradient; both variants match the FP64 oracle
+    on two positions."""
+    from mset_amd import exitwaves, lib
+    wl = exitwaves.make_workload(64)
+    out = exitwaves.run(64, repeat=1, wl=wl)
+    print('256^3 store vs recompute:', {k: out[k] for k in ('wave_memory_ratio', 'scratch_memory_ratio', 'time_ratio',
+                                                             'recompute_vs_store')}, flush=True)
+    assert wl.L == 256 and out['wave_memory_ratio'] == (wl.L + 2) / 5
+    assert out['variants']['store']['resolved'] == lib.EXIT_WAVES_STORE
+    assert out['variants']['recompute']['resolved'] == lib.EXIT_WAVES_RECOMPUTE
+    assert out['recompute_vs_store']['dV_rel'] < 1e-3 and out['recompute_vs_store']['loss_rel'] < 1e-6
+    V0 = (0.5 * wl.truth).astype(np.float64)
+    xl, yl, M = wl.xlists[0][:2], wl.ylists[0][:2], wl.measured[0][:2]
+    odV, odp, _, oep = O.grad_fixed(V0, wl.probe, wl.prop, np.conj(wl.prop), xl, yl, M, wl.binning, wl.sigma, wl.dx)
+    for mode in (lib.EXIT_WAVES_STORE, lib.EXIT_WAVES_RECOMPUTE):
+        with lib.Context(0) as ctx:
+            ctx.upload(V0, wl.probe, wl.prop, xl, yl, M, wl.sigma, wl.dx, slice_binning=wl.binning, n_scan=(2, 1))
+            ctx.set_exit_waves(mode)
+            dV, dp, _, err = ctx.grad()
+        print('  exit waves %d vs FP64: dV %.2e dprobe %.2e loss %.2e'
+              % (mode, rel(dV, odV), rel(dp, odp), abs(err / (oep.sum() / 2) - 1)), flush=True)
+        assert rel(dV, odV) < 1e-3 and rel(dp, odp) < 1e-3 and abs(err / (oep.sum() / 2) - 1) < 1e-4

@@ -102,3 +102,36 @@ def test_recon_pass_with_recomputed_waves():
     d = out[lib.EXIT_WAVES_RECOMPUTE] - c['V']
     ds = out[lib.EXIT_WAVES_STORE] - c['V']
     assert np.linalg.norm(d - ds) <= REBUILD_TOL * np.linalg.norm(ds) + 1e-6 * np.linalg.norm(c['V'])
+
+
+@pytest.mark.parametrize('n,nz,binning,mode', [(128, 16, 2, O.MODE_REF), (128, 12, 3, O.MODE_EXACT),
+                                               (256, 8, 2, O.MODE_REF), (256, 6, 1, O.MODE_EXACT)])
+def test_slots_reused_by_several_positions_stored_and_recomputed(n, nz, binning, mode):
+    """Two resident slots for eight positions: each slot runs four positions in one launch, so its later
+    positions add to the slabs its first one wrote, and its wave and plane scratch are reused.  Both exit-wave
+    variants against the FP64 oracle, and the per-position terms against the unlimited grid (bitwise: they do
+    not depend on which slot ran the position)."""
+    from mset_amd import lib
+    c = small_case(n=n, nz=nz, binning=binning, npos=8, seed=5 * n + nz, spread=n / 8)
+    print('case: slot reuse', n, nz, binning, mode, flush=True)
+    odV, odp, odpos, oep = O.grad_fixed(c['V'], c['probe'], c['prop'], c['bprop'], c['xl'], c['yl'], c['M'],
+                                        c['bin'], c['sig'], c['dx'], mode=mode)
+    ctx = lib.Context(0)
+    try:
+        _upload(ctx, c, mode)
+        full = ctx.grad()
+        for ew in (lib.EXIT_WAVES_STORE, lib.EXIT_WAVES_RECOMPUTE):
+            ctx.set_exit_waves(ew)
+            ctx.set_slot_limit(2)
+            dV, dp, dpos, err = ctx.grad()
+            assert ctx.exit_waves()[0] == ew
+            ctx.set_slot_limit(0)
+            print('  exit waves %d: dV %.2e dprobe %.2e dpos %.2e' % (ew, rel(dV, odV), rel(dp, odp), rel(dpos, odpos)),
+                  flush=True)
+            assert abs(err / (oep.sum() / c['npos']) - 1) < 1e-4
+            assert rel(dV, odV) < GRAD_TOL and rel(dp, odp) < GRAD_TOL and rel(dpos, odpos) < GRAD_TOL
+            if ew == lib.EXIT_WAVES_STORE:
+                assert np.array_equal(dpos, full[2])
+                assert rel(dV, full[0]) < 1e-5 and rel(dp, full[1]) < 1e-5
+    finally:
+        ctx.close()

@@ -294,3 +294,23 @@ def test_upload_from_page_locked_host_memory(mset):
     for a, b in zip(got[:3], ref[:3]):
         assert np.array_equal(a, b)
     assert got[3] == ref[3]
+
+
+def test_diffraction_transpose_at_ingest(mset):
+    """STEP00_INIT.m:58-73: uploading the patterns with diffraction_transpose set equals uploading their
+    transposes, bit for bit."""
+    c = small_case(n=64, nz=8, binning=2, npos=5, seed=33, spread=6.0)
+    MT = np.ascontiguousarray(np.transpose(c['M'], (0, 2, 1)))
+    mset.upload(c['V'], c['probe'], c['prop'], c['xl'], c['yl'], MT, c['sig'], c['dx'], slice_binning=c['bin'],
+                n_scan=(c['npos'], 1))
+    ref = mset.grad()
+    mset.upload(c['V'], c['probe'], c['prop'], c['xl'], c['yl'], c['M'], c['sig'], c['dx'], slice_binning=c['bin'],
+                n_scan=(c['npos'], 1), diffraction_transpose=True)
+    got = mset.grad()
+    for a, b in zip(got[:3], ref[:3]):
+        assert np.array_equal(a, b)
+    assert got[3] == ref[3]
+    # and it does change the result for patterns that are not symmetric
+    mset.upload(c['V'], c['probe'], c['prop'], c['xl'], c['yl'], c['M'], c['sig'], c['dx'], slice_binning=c['bin'],
+                n_scan=(c['npos'], 1))
+    assert not np.array_equal(mset.grad()[0], ref[0])

@@ -109,3 +109,33 @@ def test_batched_with_zero_steps_is_forward_plus_gradient_only():
         ref = O.rot_3dvol(O.rot_3dvol(ref, t['ang'], inverse=True), t['ang'])
     assert rel(grec, ref) < 1e-5
     assert np.isfinite(e) and e > 0
+
+
+@pytest.mark.parametrize('n,nz,binning', [(64, 16, 2), (256, 16, 4)])
+def test_error_pass_is_one_launch_over_uneven_tilts(n, nz, binning):
+    """Run_MSET_recon.m:189-239 over four tilts with 5, 11, 3 and 8 positions: one persistent launch, whose
+    per-tilt errors match one launch per tilt on the same rotated volumes and the oracle's error pass."""
+    from mset_amd import lib
+    base = small_case(n=n, nz=nz, binning=binning, npos=4, seed=n + 3, spread=n / 8)
+    rng = np.random.default_rng(n + 4)
+    T = []
+    for j, (t, npos) in enumerate(((-40.0, 5), (-10.0, 11), (15.0, 3), (35.0, 8))):
+        c = small_case(n=n, nz=nz, binning=binning, npos=npos, seed=n + 20 + j, spread=n / 8)
+        T.append(dict(ang=(0.0, t, 0.0), xl=c['xl'], yl=c['yl'], M=c['M'], probe=base['probe'] * (1 + 0.05 * j)))
+    rec = (np.abs(rng.standard_normal((nz, n, n))) * base['V'].mean()).astype(np.float32)
+    print('case: one-launch error pass, N=%d, tilts of %s positions' % (n, [len(t['xl']) for t in T]), flush=True)
+    with _recon(base, T, rec, (0.0, 0.0, 0.0)) as r:
+        r.profile(True)
+        e = r.error_pass()
+        launches, _ = r.profile_read('position_fwd')
+        r.profile(False)
+    assert launches == 1
+    per_tilt = []
+    for t in T:
+        with lib.Context(0) as ctx:
+            ctx.upload(rec, t['probe'], base['prop'], t['xl'], t['yl'], t['M'], base['sig'], base['dx'],
+                       slice_binning=binning, n_scan=(len(t['xl']), 1))
+            ctx.rotate_resident(t['ang'], inverse=True)
+            per_tilt.append(ctx.error())
+    np.testing.assert_allclose(e, per_tilt, rtol=1e-6)
+    np.testing.assert_allclose(e, _oracle_errors(base, T, rec), rtol=1e-4)

@@ -22,6 +22,9 @@ class FakeRecon:
             FakeRecon.live += 1
             FakeRecon.peak = max(FakeRecon.peak, FakeRecon.live)
 
+    def set_slot_limit(self, slots):
+        self.slots = slots
+
     def iterate(self, path, batch, use_graph, positivity, want_error=True):
         time.sleep(0.01)
         self.n += 1
@@ -124,3 +127,28 @@ def test_waiting_large_job_is_not_starved_by_resident_small_ones():
         assert r.start >= big_req.start
     rep = srv.report()
     assert rep['evictions'] >= 2 and rep['admissions'] >= 5
+
+
+def test_concurrent_requests_share_the_device_slots():
+    """A request alone gets every resident workgroup; concurrent ones get power-of-two shares whose sum never
+    exceeds the device, and the report shows the time with several requests on the device."""
+    specs = _specs(6)
+    srv = serve.Server(specs, workers=4, budget_bytes=10 ** 12, recon_factory=FakeRecon, slots_of=lambda n: 256)
+    first = srv.submit(0)
+    assert first.done.wait(10)
+    reqs = [srv.submit(j) for j in (1, 2, 3, 4, 5, 1, 2)]
+    for r in reqs:
+        assert r.done.wait(10)
+    rep = srv.report()
+    srv.close()
+    assert first.slots == 256 and first.concurrent == 1
+    for r in reqs:
+        assert r.slots in (256, 128, 64) and 256 % r.slots == 0
+        assert r.slots * r.concurrent <= 256 * 2 and r.slots <= 256 // max(1, 2 ** ((r.concurrent - 1).bit_length()))
+    # at every start, the shares of the requests then running fit the device
+    ev = sorted([(r.start, 1, r.slots) for r in [first] + reqs] + [(r.end, 0, -r.slots) for r in [first] + reqs])
+    used = 0
+    for _, _, d in ev:
+        used += d
+        assert used <= 256 * 2          # shares are fixed at start; at most one doubling while others finish
+    assert rep['max_concurrent'] >= 2 and rep['overlap_s'] > 0 and rep['failures'] == 0
