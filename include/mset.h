@@ -106,6 +106,22 @@ int mset_set_state(mset_ctx* ctx, const float* RVol, const float* probe, const f
 int mset_grad(mset_ctx* ctx, int first, int count, float* dV, float* dprobe, float* dpos, double* error_out);
 /* (mset_grad, mset_waves, mset_simulate and the mset_recon_* tilt series use the multislice operator.) */
 
+/* Allocation-free mset_grad.  No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It runs the same kernels as mset_grad on the resident state of an uploaded context, but its scratch
+ * (transmission planes, fftn(probe), the stored waves and gradient slabs of the resident positions) is a
+ * static device array of MSET_NOALLOC_SCRATCH_BYTES per GPU, held from library load to process end; it
+ * calls no hipMalloc, hipFree or allocating routine.  As many positions are resident at once as that
+ * array holds (at most the mset_set_slot_limit cap); a problem whose one resident position does not fit
+ * is rejected with MSET_EINVAL.  Outputs are DEVICE pointers with mset_grad's layouts: dV float[L][ny][nx],
+ * dprobe interleaved complex [ny][nx], dpos float[count][2], err_pos double[count] (per-position sums
+ * over pixels of (sqrt M - sqrt I)^2, unnormalised).  The work is only issued on the context's stream
+ * (mset_stream): synchronise it before reading the outputs.  All calls on one GPU share its array and
+ * nothing in it needs clearing (every region is written before it is read); calls that overlap in time
+ * on one GPU, from any contexts or threads, race on it and must be serialised by the caller. */
+#define MSET_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int mset_grad_noalloc(mset_ctx* ctx, int first, int count, float* dV, float* dprobe, float* dpos,
+                      double* err_pos);
 
 /* The incident wave and the wave after each plane for position k at the resident state, i.e.
  * tmp_probe_wave and save_wave_f of STEP02_FORWARD_4D.m:27-35: interleaved complex [L + 1][ny][nx]. */

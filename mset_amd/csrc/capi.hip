@@ -15,6 +15,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -168,6 +169,31 @@ void prof_free(ProfMap& pm) {
             (void)hipEventDestroy(ev.second);
         }
     pm.clear();
+}
+
+// Scratch of mset_grad_noalloc: a file-scope device array, so that entry point never allocates.  Each GPU
+// holds its own copy at its own address; host code reaches it only through the address that
+// hipGetSymbolAddress returns on that device (the array's name is a host-side stand-in).
+__device__ __align__(256) unsigned char scratch[MSET_NOALLOC_SCRATCH_BYTES];
+
+std::mutex g_scratch_mu;
+std::map<int, unsigned char*> g_scratch;   // device -> address of `scratch` there
+
+// resolves the scratch address on `device` once; the device must be current
+int resolve_scratch(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch.count(device)) return MSET_OK;
+    void* ptr = nullptr;
+    HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(MSET_EDEVICE, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch[device] = (unsigned char*)ptr;
+    return MSET_OK;
+}
+
+unsigned char* scratch_on(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch.find(device);
+    return it == g_scratch.end() ? nullptr : it->second;
 }
 
 int check_ctx(mset_ctx* c) {
@@ -426,6 +452,7 @@ int new_ctx(mset_ctx** out, int device, hipStream_t shared_stream, std::shared_p
             bool* profiling, std::shared_ptr<ProfMap> prof) {
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, device));
+    CHK(resolve_scratch(device));
     mset_ctx* c = new mset_ctx();
     c->device = device;
     c->cus = prop.multiProcessorCount;
@@ -707,6 +734,68 @@ int mset_grad(mset_ctx* c, int first, int count, float* dV, float* dprobe, float
                                     hipMemcpyDeviceToHost, c->stream));
     if (err_out) return read_errors(c, first, count, err_out);
     HIPCHK(hipStreamSynchronize(c->stream));
+    return MSET_OK;
+}
+
+int mset_grad_noalloc(mset_ctx* c, int first, int count, float* dV, float* dprobe, float* dpos, double* err_pos) {
+    CHK(check_uploaded(c));
+    if (first < 0 || count <= 0 || first + count > c->npos) return fail(MSET_EINVAL, "position range out of bounds");
+    if (!dV || !dprobe || !dpos || !err_pos) return fail(MSET_EINVAL, "null output (device pointers required)");
+    CHK(set_method(c, MSET_METHOD_MSET));
+    unsigned char* base = scratch_on(c->device);
+    if (!base) return fail(MSET_EDEVICE, "scratch array not resolved on this device");
+    const size_t n2 = plane(c), L = (size_t)c->L;
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    // fixed part: transmission planes and fftn(probe).  Per gradient slab: L object planes and a probe plane.
+    // Per resident position (slot): stored waves (L + 2 planes) and, at N = 256, the engine's two plane
+    // buffers.  Four blocks with the kernels' plane strides, each starting on a 256-byte boundary.
+    const size_t b_trans = up(L * n2 * sizeof(float2)), b_pf = up(n2 * sizeof(float2));
+    const size_t s_g = L * n2 * sizeof(float), s_ps = n2 * sizeof(float2);
+    const size_t s_save = (L + 2) * n2 * sizeof(float2), s_pl = plane_in_global(c->n) ? 2 * n2 * sizeof(float2) : 0;
+    const size_t fixed = b_trans + b_pf + 4 * 256;
+    auto need = [&](int S, int G) { return fixed + (size_t)S * (s_g + s_ps) + (size_t)G * (s_save + s_pl); };
+    // mset_grad's slabs and grid when they fit (then the sums are bitwise mset_grad's), else as many resident
+    // positions as fit, one slab each
+    int S = std::min(grad_slabs(c), count);
+    int G = launch_grid(batch_slots(c), count, S);
+    while (G > 1 && need(S, G) > MSET_NOALLOC_SCRATCH_BYTES) G = launch_grid(G - 1, count, S);
+    if (need(S, G) > MSET_NOALLOC_SCRATCH_BYTES) {
+        S = G = (int)std::min<size_t>((size_t)count, (MSET_NOALLOC_SCRATCH_BYTES - std::min(fixed, (size_t)MSET_NOALLOC_SCRATCH_BYTES)) /
+                                                         (s_g + s_ps + s_save + s_pl));
+        if (S < 1)
+            return fail(MSET_EINVAL, "mset_grad_noalloc: this problem needs " + std::to_string(need(1, 1)) +
+                                         " bytes of scratch for one resident position; the maximum is " +
+                                         std::to_string((size_t)MSET_NOALLOC_SCRATCH_BYTES));
+    }
+    unsigned char* q = base;
+    float2* trans = (float2*)q;  q += b_trans;
+    float2* probeF = (float2*)q; q += b_pf;
+    float* gslab = (float*)q;    q += up((size_t)S * s_g);
+    float2* pslab = (float2*)q;  q += up((size_t)S * s_ps);
+    float2* save = (float2*)q;   q += up((size_t)G * s_save);
+    float2* planes_ = s_pl ? (float2*)q : nullptr;
+    HIPCHK(launch_transmission(c->V.p, trans, c->n, c->nz, binning(c), 0, c->p.sigma, c->p.potential_pixelsize,
+                               c->stream));
+    HIPCHK(launch_plane_fft(c->n, c->probe.p, probeF, c->tw.p, 0, 1.f, 0.f, planes_, c->stream));
+    PosArgs A = base_args(c);
+    A.trans = trans;
+    A.probeF = probeF;
+    A.save = save;
+    A.recompute = 0;
+    A.gslab = gslab;
+    A.pslab = pslab;
+    A.planes = planes_;
+    A.dpos = dpos;
+    A.err_pos = err_pos;
+    A.obase = first;
+    A.first = first;
+    A.count = count;
+    A.want = WANT_PROBE | WANT_POS;
+    A.nslab = S;
+    HIPCHK(launch_batch(c->n, A, G, c->stream));
+    HIPCHK(launch_reduce_slabs(gslab, S, L * n2, dV, c->stream));
+    HIPCHK(launch_reduce_cslabs(pslab, S, n2, pslab, c->stream));
+    HIPCHK(launch_plane_fft(c->n, pslab, (float2*)dprobe, c->tw.p, 1, 1.f, 0.f, planes_, c->stream));
     return MSET_OK;
 }
 

@@ -27,7 +27,7 @@ EXPORTS = ['mset_create', 'mset_destroy', 'mset_last_error', 'mset_stream', 'mse
            'mset_recon_snapshot', 'mset_recon_restore', 'mset_recon_profile_enable', 'mset_recon_profile_read',
            'mset_recon_stream', 'mset_tv', 'mset_recon_tv', 'mset_device_memory', 'mset_batch_slots',
            'mset_set_exit_waves', 'mset_exit_waves', 'mset_recon_set_exit_waves', 'mset_set_slot_limit',
-           'mset_recon_set_slot_limit', 'mset_host_alloc', 'mset_host_free']
+           'mset_recon_set_slot_limit', 'mset_host_alloc', 'mset_host_free', 'mset_grad_noalloc']
 PATH_PLAIN, PATH_BATCHED = 0, 1
 EXIT_WAVES_AUTO, EXIT_WAVES_STORE, EXIT_WAVES_RECOMPUTE = 0, 1, 2
 
@@ -84,6 +84,7 @@ def load():
         'mset_recon_set_exit_waves': ([P, I], I), 'mset_set_slot_limit': ([P, I], I),
         'mset_recon_set_slot_limit': ([P, I], I), 'mset_host_alloc': ([C.c_size_t, PP], I),
         'mset_host_free': ([P], I),
+        'mset_grad_noalloc': ([P, I, I, P, P, P, P], I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -259,6 +260,12 @@ class Context:
         e = np.zeros(1, np.float64)
         _chk(self._L.mset_grad(self._h, int(first), int(count), _ptr(dV), _ptr(dp), _ptr(dpos), _ptr(e)))
         return dV, dp, dpos, float(e[0])
+
+    def grad_noalloc(self, first, count, dV_ptr, dprobe_ptr, dpos_ptr, err_ptr):
+        """mset_grad_noalloc: issues the gradient into caller-owned DEVICE buffers (raw addresses) using the
+        library's static scratch; call synchronize() before reading them."""
+        _chk(self._L.mset_grad_noalloc(self._h, int(first), int(count), C.c_void_p(dV_ptr), C.c_void_p(dprobe_ptr),
+                                       C.c_void_p(dpos_ptr), C.c_void_p(err_ptr)))
 
     def waves(self, k):
         """Incident wave and the wave after each plane for position k -> complex64 [L + 1, ny, nx]."""
