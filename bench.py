@@ -42,9 +42,11 @@ WORKLOADS = {
           'over all tilts per step',
 }
 # object, probe and position step sizes of the timed pass.  The object step is the example's (main.m:42);
-# the probe and position steps move the probe by about a percent and the positions by about 0.01 px per
-# pass on these workloads.  Nonzero, they make the kernels form the probe and position terms.
-STEPS_ALL = (100.0, 0.02, 1e8)
+# the probe and position steps are the largest of a sweep on five C3 tilts (probe 1e-3 / 1e-2, positions
+# 1e6 / 1e7) that kept the error finite and falling (1e-2 diverged; 1e6 moved positions by thousands of
+# pixels), the position step cut to keep moves well under a pixel.  Nonzero, they make the kernels form
+# the probe and position terms on every position, as the reference does.
+STEPS_ALL = (100.0, 1e-3, 10.0)
 STEPS_OBJECT = (100.0, 0.0, 0.0)
 
 
@@ -149,8 +151,10 @@ def main():
         full = lib.batch_slots(wl.P)
         tuned = autotune.tune_recon(r, batches=batches, slots=[full, full // 2], log=log)
         args.batch = tuned['batch']
-        tuned = dict(batch=tuned['batch'], slots=tuned['slots'],
-                     pass_ms={'%d/%d' % (sl, b): round(dt, 2) for sl, b, dt, _ in tuned['table']})
+        tuned = dict(engine=['auto (separable at 256)', 'general'][tuned['engine']], batch=tuned['batch'],
+                     slots=tuned['slots'],
+                     pass_ms={'engine %d / slots %d / batch %d' % (e, sl, b): round(dt, 2)
+                              for e, sl, b, dt, _ in tuned['table']})
     else:
         args.batch = batches[0]
     log('batched path: batch %d, graph %s, steps %s' % (args.batch, use_graph, STEPS_ALL))

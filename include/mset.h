@@ -143,6 +143,15 @@ int mset_exit_waves(mset_ctx* ctx, int* resolved, size_t* bytes_per_slot);
  * 0 (the default) = every workgroup the device keeps resident (mset_batch_slots).  A tuning knob with
  * no counterpart in the reference: fewer slots hold less scratch and may keep more of it in cache. */
 int mset_set_slot_limit(mset_ctx* ctx, int slots);
+/* Plane routine of the 256 x 256 kernels.  AUTO (the default): when the propagator factors as
+ * p(kx, ky) = px(kx) py(ky) (the Fresnel propagator of Func_generate_propagator.m does, checked at upload)
+ * the forward kernels and the reference-pairing batched gradient with stored waves run the separable
+ * routine, which applies the two 1D half-steps of neighbouring planes in one pass over the plane (half the
+ * passes); GENERAL always runs the 2D routine.  Both agree with the FP64 oracle to the same tolerances; a
+ * tuning knob with no counterpart in the reference. */
+#define MSET_ENGINE_AUTO 0
+#define MSET_ENGINE_GENERAL 1
+int mset_set_engine(mset_ctx* ctx, int engine);
 
 /* Mini-batch variant of 'run_mset': positions are taken in order in batches of `batch`; each batch's
  * gradient is formed at the state left by the previous batch and then applied.  batch == 1 equals
@@ -210,6 +219,8 @@ int mset_recon_tv(mset_recon* r, float lambda, int n_iter, double* tv_error);
 int mset_recon_set_exit_waves(mset_recon* r, int mode);
 /* slot cap of the tilt series' batched passes (as mset_set_slot_limit) */
 int mset_recon_set_slot_limit(mset_recon* r, int slots);
+/* mset_set_engine for the series' passes (drops a captured graph when it changes). */
+int mset_recon_set_engine(mset_recon* r, int engine);
 /* BLS rollback (Run_MSET_recon.m:113-115, 245-247) */
 int mset_recon_snapshot(mset_recon* r);
 int mset_recon_restore(mset_recon* r);

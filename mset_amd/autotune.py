@@ -10,8 +10,9 @@ Tunables (both leave the kernels' arithmetic unchanged):
   batch  positions per launch of an update pass (the mini-batch size), from the range the caller
          allows: it sets how many launches, reductions and updates a pass makes and how well the last
          wave of workgroups fills the device.
-The plane engines themselves (radix, LDS tile, workgroup shape) are fixed per plane size at compile
-time; the grid is what remains to tune at run time.
+  engine the plane routine at 256 x 256 (mset_set_engine): the separable-propagator routine, one pass over
+         the plane per propagation step, or the general 2D routine, two passes per step.
+The radix plans and LDS tiles of the routines are fixed per plane size at compile time.
 
 Each candidate runs one update pass from a snapshot of the tilt series, which is restored afterwards,
 so tuning leaves the reconstruction unchanged.  The kernels are timed with HIP events on the series'
@@ -39,53 +40,69 @@ def slot_candidates(n, fracs=(1.0, 0.5, 0.25), device=0):
     return out
 
 
-def _key(recon, batches, slots):
+def engine_candidates(n):
+    """Plane routines worth timing at plane size n (two exist at 256)."""
+    return (lib.ENGINE_AUTO, lib.ENGINE_GENERAL) if n == 256 else (lib.ENGINE_AUTO,)
+
+
+def _key(recon, batches, slots, engines):
     return json.dumps(dict(shape=list(recon.shape), npos=list(recon.npos), batches=list(batches),
-                           slots=list(slots)), sort_keys=True)
+                           slots=list(slots), engines=list(engines)), sort_keys=True)
 
 
-def tune_recon(recon, batches=(64, 128, 256, 512), slots=None, cache_path=None, log=None):
-    """Time one batched update pass per (slots, batch) candidate on `recon` and leave the fastest grid
-    set on it.  Returns dict(batch, slots, ms, table=[(slots, batch, pass ms, position-kernel ms)])."""
+def _apply(recon, best):
+    recon.set_engine(best.get('engine', lib.ENGINE_AUTO))
+    recon.set_slot_limit(best['slots'])
+
+
+def tune_recon(recon, batches=(64, 128, 256, 512), slots=None, engines=None, cache_path=None, log=None):
+    """Time one batched update pass per (engine, slots, batch) candidate on `recon` and leave the fastest
+    engine and grid set on it.  Returns dict(engine, batch, slots, ms, table=[(engine, slots, batch, pass ms,
+    position-kernel ms)])."""
     if slots is None:
         slots = slot_candidates(recon.shape[1])
-    key = _key(recon, batches, slots)
+    if engines is None:
+        engines = engine_candidates(recon.shape[1])
+    key = _key(recon, batches, slots, engines)
     if key in _CACHE:
         best = _CACHE[key]
-        recon.set_slot_limit(best['slots'])
+        _apply(recon, best)
         return best
     if cache_path and os.path.exists(cache_path):
         with open(cache_path) as fh:
             disk = json.load(fh)
         if key in disk:
             _CACHE[key] = disk[key]
-            recon.set_slot_limit(disk[key]['slots'])
+            _apply(recon, disk[key])
             return disk[key]
     recon.snapshot()
     table = []
     try:
-        for s in slots:
-            recon.set_slot_limit(s)
-            recon.iterate(lib.PATH_BATCHED, max(batches), use_graph=False, want_error=False)   # sizes scratch
-            recon.restore()
-            for b in batches:
-                recon.profile(True)
-                t0 = time.perf_counter()
-                recon.iterate(lib.PATH_BATCHED, b, use_graph=False, want_error=False)
-                dt = (time.perf_counter() - t0) * 1e3
-                _, kms = recon.profile_read('position_batch')
-                recon.profile_read('rotate')                # consumed here, not left for the caller
-                recon.profile(False)
+        for e in engines:
+            recon.set_engine(e)
+            for s in slots:
+                recon.set_slot_limit(s)
+                recon.iterate(lib.PATH_BATCHED, max(batches), use_graph=False, want_error=False)   # sizes scratch
                 recon.restore()
-                table.append((s, b, dt, kms))
-                if log:
-                    log('autotune: slots %d batch %d: pass %.2f ms, position kernels %.2f ms' % (s, b, dt, kms))
+                for b in batches:
+                    recon.profile(True)
+                    t0 = time.perf_counter()
+                    recon.iterate(lib.PATH_BATCHED, b, use_graph=False, want_error=False)
+                    dt = (time.perf_counter() - t0) * 1e3
+                    _, kms = recon.profile_read('position_batch')
+                    recon.profile_read('rotate')                # consumed here, not left for the caller
+                    recon.profile(False)
+                    recon.restore()
+                    table.append((e, s, b, dt, kms))
+                    if log:
+                        log('autotune: engine %d slots %d batch %d: pass %.2f ms, position kernels %.2f ms'
+                            % (e, s, b, dt, kms))
     finally:
         recon.profile(False)
         recon.restore()
-    s, b, dt, kms = min(table, key=lambda r: r[2])
-    best = dict(batch=b, slots=s, ms=dt, table=table)
-    recon.set_slot_limit(s)
+    e, s, b, dt, kms = min(table, key=lambda r: r[3])
+    best = dict(engine=e, batch=b, slots=s, ms=dt, table=table)
+    _apply(recon, best)
     _CACHE[key] = best
     if cache_path:
         disk = {}

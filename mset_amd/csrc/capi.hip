@@ -87,6 +87,7 @@ struct Workspace {
     DevBuf<float> gslab, G;
     int exit_waves = MSET_EXIT_WAVES_AUTO;   // requested exit-wave variant of the batched kernels
     int slot_limit = 0;                      // cap on the persistent grid of the batched kernels (0: none)
+    int engine = MSET_ENGINE_AUTO;           // plane routine of the 256 x 256 kernels
     bool recompute = false;                  // the variant of the last batched call
     size_t slot_bytes = 0;                   // its per-slot scratch (waves + gradient slab)
     RotWork rot;
@@ -109,6 +110,8 @@ struct mset_ctx {
     int method = MSET_METHOD_MSET;   // operator of the current call (MSET, or SSET = one plane of depth nz)
     bool uploaded = false;
     bool prop_unitary = false;       // |prop| = 1 everywhere (to 1e-5): the forward sweep can be inverted
+    bool prop_sep = false;           // prop(kx, ky) = px(kx) py(ky) (to 2e-6): the separable kernels apply
+    DevBuf<float2> p1d;              // px, py (N each) when prop_sep
     DevBuf<float> V, sqrtM, x, y, dpos, Mtmp;
     DevBuf<float2> probe, prop, tw;
     DevBuf<double> err;
@@ -261,8 +264,17 @@ PosArgs base_args(mset_ctx* c) {
     A.pslab = c->ws->pslab.p;
     A.dpos = c->dpos.p;
     A.planes = plane_in_global(c->n) ? c->ws->planes.p : nullptr;
+    A.sep = 0;
+    A.p1d = c->p1d.p;
     return A;
 }
+
+// The separable-propagator kernels (the 256 x 256 engine) serve the forward-only launches and the batched
+// gradient with the reference pairing and stored waves; everything else runs the general kernels.
+bool sep_fwd(const mset_ctx* c) {
+    return plane_in_global(c->n) && c->prop_sep && !sset(c) && c->ws->engine != MSET_ENGINE_GENERAL;
+}
+bool sep_batch(const mset_ctx* c) { return sep_fwd(c) && c->p.grad_mode == GRAD_REF && !c->ws->recompute; }
 
 // workgroups of the batched kernels the device keeps resident at once (1 per CU at N = 128)
 int device_slots(mset_ctx* c) {
@@ -296,7 +308,8 @@ int launch_grid(int want, int count, int S) {
 
 int ensure_fixed_buffers(mset_ctx* c) {
     Workspace& W = *c->ws;
-    CHK(W.trans.ensure(plane(c) * c->L, "transmission"));
+    // the separable kernels keep the transmission in two layouts (forward and adjoint passes)
+    CHK(W.trans.ensure(plane(c) * c->L * (plane_in_global(c->n) ? 2 : 1), "transmission"));
     CHK(W.probeF.ensure(plane(c), "probe spectrum"));
     // the position kernels of the largest planes keep the plane in global memory: 2 planes per slot
     if (plane_in_global(c->n)) CHK(W.planes.ensure((size_t)batch_slots(c) * 2 * plane(c), "plane buffers"));
@@ -342,11 +355,16 @@ int ensure_batch_buffers(mset_ctx* c) {
     return MSET_OK;
 }
 
-// transmission planes + fftn(probe) for the fixed-state kernels
-int prepare_fixed(mset_ctx* c) {
+// transmission planes + fftn(probe) for the fixed-state kernels; sep: 1 = forward layouts of the separable
+// kernels, 2 = forward and adjoint layouts
+int prepare_fixed(mset_ctx* c, int sep = 0) {
     CHK(ensure_fixed_buffers(c));
-    HIPCHK(launch_transmission(c->V.p, c->ws->trans.p, c->n, c->nz, binning(c), sset(c) ? 1 : 0, c->p.sigma,
-                               c->p.potential_pixelsize, c->stream));
+    if (sep)
+        HIPCHK(launch_transmission_sep(c->V.p, c->ws->trans.p, sep == 2 ? c->ws->trans.p + plane(c) * c->L : nullptr,
+                                       c->n, c->nz, binning(c), c->p.sigma, c->p.potential_pixelsize, c->stream));
+    else
+        HIPCHK(launch_transmission(c->V.p, c->ws->trans.p, c->n, c->nz, binning(c), sset(c) ? 1 : 0, c->p.sigma,
+                                   c->p.potential_pixelsize, c->stream));
     HIPCHK(launch_plane_fft(c->n, c->probe.p, c->ws->probeF.p, c->tw.p, 0, 1.f, 0.f, c->ws->planes.p, c->stream));
     return MSET_OK;
 }
@@ -387,8 +405,10 @@ int issue_plain(mset_ctx* c) {
 }
 
 int issue_fwd(mset_ctx* c, float* inten) {
-    CHK(prepare_fixed(c));
+    const bool sep = sep_fwd(c);
+    CHK(prepare_fixed(c, sep ? 1 : 0));
     PosArgs A = base_args(c);
+    A.sep = sep ? 1 : 0;
     A.first = 0;
     A.count = c->npos;
     A.inten = inten;
@@ -402,7 +422,8 @@ int issue_fwd(mset_ctx* c, float* inten) {
 // positions += s3 dpos).  want: the terms to form (the gradient request forms all of them).
 int issue_grad(mset_ctx* c, int first, int count, bool apply, int want) {
     CHK(ensure_batch_buffers(c));
-    CHK(prepare_fixed(c));
+    const bool sep = sep_batch(c);
+    CHK(prepare_fixed(c, sep ? 2 : 0));
     Workspace& W = *c->ws;
     const int S = grad_slabs(c);
     const int grid = launch_grid(c->slots, count, S);
@@ -412,12 +433,15 @@ int issue_grad(mset_ctx* c, int first, int count, bool apply, int want) {
     A.count = count;
     A.want = want;
     A.nslab = S;
+    A.sep = sep ? 1 : 0;
+    A.trans_b = W.trans.p + plane(c) * c->L;
     // every used slab receives at least one position, and its first one writes it
     {
         Timed t(c, "position_batch");
         HIPCHK(launch_batch(c->n, A, grid, c->stream));
     }
     HIPCHK(launch_reduce_slabs(W.gslab.p, used, (size_t)planes(c) * plane(c), W.G.p, c->stream));
+    if (sep) HIPCHK(launch_untranspose_odd(W.G.p, c->n, c->L, c->stream));
     if (want & WANT_PROBE) {
         // in place into slab 0 (each element is read and written by the same thread), then PG = ifftn
         HIPCHK(launch_reduce_cslabs(W.pslab.p, used, plane(c), W.pslab.p, c->stream));
@@ -531,6 +555,31 @@ int upload_impl(mset_ctx* c, const mset_params* p, const float* RVol, const floa
     double dev = 0.0;
     for (const float2& q : pu) dev = std::max(dev, std::fabs((double)q.x * q.x + (double)q.y * q.y - 1.0));
     c->prop_unitary = dev <= 1e-5;
+    // Separable propagator (Func_generate_propagator.m: exp(-i pi lambda dz (qx^2 + qy^2))): with p(ky, kx) =
+    // pu[kx][ky], px(kx) = p(0, kx) and py(ky) = p(ky, 0) / p(0, 0) reproduce every element to FP32 rounding
+    // (f1 stays alive until the synchronisation below: its copy is asynchronous)
+    std::vector<float2> f1(2 * (size_t)n);
+    {
+        const double r00 = pu[0].x, i00 = pu[0].y, m00 = r00 * r00 + i00 * i00;
+        bool sep = m00 > 0.25;
+        for (int kx = 0; kx < n; ++kx) f1[kx] = pu[(size_t)kx * n];
+        for (int ky = 0; ky < n && sep; ++ky) {
+            const double a = pu[ky].x, b = pu[ky].y;                      // p(ky, 0) / p(0, 0)
+            f1[n + ky] = make_float2((float)((a * r00 + b * i00) / m00), (float)((b * r00 - a * i00) / m00));
+        }
+        double worst = 0.0;
+        for (int kx = 0; kx < n && sep; ++kx)
+            for (int ky = 0; ky < n; ++ky) {
+                const float2 u = f1[kx], w = f1[n + ky], q = pu[(size_t)kx * n + ky];
+                const double re = (double)u.x * w.x - (double)u.y * w.y, im = (double)u.x * w.y + (double)u.y * w.x;
+                worst = std::max(worst, std::hypot(re - q.x, im - q.y));
+            }
+        c->prop_sep = sep && worst <= 2e-6;
+        if (c->prop_sep) {
+            CHK(c->p1d.ensure(2 * (size_t)n, "propagator factors"));
+            HIPCHK(hipMemcpyAsync(c->p1d.p, f1.data(), sizeof(float2) * 2 * n, hipMemcpyHostToDevice, c->stream));
+        }
+    }
     std::vector<float2> tw;
     twiddles(n, tw);
     if (RVol) HIPCHK(hipMemcpyAsync(c->V.p, RVol, sizeof(float) * n2 * c->nz, hipMemcpyHostToDevice, c->stream));
@@ -749,7 +798,9 @@ int mset_grad_noalloc(mset_ctx* c, int first, int count, float* dV, float* dprob
     // fixed part: transmission planes and fftn(probe).  Per gradient slab: L object planes and a probe plane.
     // Per resident position (slot): stored waves (L + 2 planes) and, at N = 256, the engine's two plane
     // buffers.  Four blocks with the kernels' plane strides, each starting on a 256-byte boundary.
-    const size_t b_trans = up(L * n2 * sizeof(float2)), b_pf = up(n2 * sizeof(float2));
+    // the separable kernels (as mset_grad picks them) keep the transmission in two layouts
+    const bool sep = sep_fwd(c) && c->p.grad_mode == GRAD_REF;
+    const size_t b_trans = up((sep ? 2 : 1) * L * n2 * sizeof(float2)), b_pf = up(n2 * sizeof(float2));
     const size_t s_g = L * n2 * sizeof(float), s_ps = n2 * sizeof(float2);
     const size_t s_save = (L + 2) * n2 * sizeof(float2), s_pl = plane_in_global(c->n) ? 2 * n2 * sizeof(float2) : 0;
     const size_t fixed = b_trans + b_pf + 4 * 256;
@@ -774,8 +825,12 @@ int mset_grad_noalloc(mset_ctx* c, int first, int count, float* dV, float* dprob
     float2* pslab = (float2*)q;  q += up((size_t)S * s_ps);
     float2* save = (float2*)q;   q += up((size_t)G * s_save);
     float2* planes_ = s_pl ? (float2*)q : nullptr;
-    HIPCHK(launch_transmission(c->V.p, trans, c->n, c->nz, binning(c), 0, c->p.sigma, c->p.potential_pixelsize,
-                               c->stream));
+    if (sep)
+        HIPCHK(launch_transmission_sep(c->V.p, trans, trans + L * n2, c->n, c->nz, binning(c), c->p.sigma,
+                                       c->p.potential_pixelsize, c->stream));
+    else
+        HIPCHK(launch_transmission(c->V.p, trans, c->n, c->nz, binning(c), 0, c->p.sigma, c->p.potential_pixelsize,
+                                   c->stream));
     HIPCHK(launch_plane_fft(c->n, c->probe.p, probeF, c->tw.p, 0, 1.f, 0.f, planes_, c->stream));
     PosArgs A = base_args(c);
     A.trans = trans;
@@ -792,8 +847,11 @@ int mset_grad_noalloc(mset_ctx* c, int first, int count, float* dV, float* dprob
     A.count = count;
     A.want = WANT_PROBE | WANT_POS;
     A.nslab = S;
+    A.sep = sep ? 1 : 0;
+    A.trans_b = trans + L * n2;
     HIPCHK(launch_batch(c->n, A, G, c->stream));
     HIPCHK(launch_reduce_slabs(gslab, S, L * n2, dV, c->stream));
+    if (sep) HIPCHK(launch_untranspose_odd(dV, c->n, c->L, c->stream));
     HIPCHK(launch_reduce_cslabs(pslab, S, n2, pslab, c->stream));
     HIPCHK(launch_plane_fft(c->n, pslab, (float2*)dprobe, c->tw.p, 1, 1.f, 0.f, planes_, c->stream));
     return MSET_OK;
@@ -847,6 +905,13 @@ int mset_set_slot_limit(mset_ctx* c, int slots) {
     CHK(check_ctx(c));
     if (slots < 0) return fail(MSET_EINVAL, "slot limit must be >= 0");
     c->ws->slot_limit = slots;
+    return MSET_OK;
+}
+
+int mset_set_engine(mset_ctx* c, int engine) {
+    CHK(check_ctx(c));
+    if (engine != MSET_ENGINE_AUTO && engine != MSET_ENGINE_GENERAL) return fail(MSET_EINVAL, "bad engine");
+    c->ws->engine = engine;
     return MSET_OK;
 }
 
@@ -1114,6 +1179,7 @@ int mset_recon_error_pass(mset_recon* r, double* errs) {
     }
     std::vector<TiltArgs> tab(T);
     std::vector<int> items(tot);
+    const bool sep = sep_fwd(c0);
     for (int j = 0; j < T; ++j) {
         mset_ctx* c = r->tilts[j];
         CHK(set_method(c, MSET_METHOD_MSET));
@@ -1121,8 +1187,12 @@ int mset_recon_error_pass(mset_recon* r, double* errs) {
         CHK(rotate_volume(c, c->V.p, r->vec, r->vec + 3, r->vec + 6, r->angles[j].data(), true));
         CHK(c->mt_trans.ensure((size_t)c->L * plane(c), "tilt transmission"));
         CHK(c->mt_probeF.ensure(plane(c), "tilt probe spectrum"));
-        HIPCHK(launch_transmission(c->V.p, c->mt_trans.p, c->n, c->nz, binning(c), 0, c->p.sigma,
-                                   c->p.potential_pixelsize, r->stream));
+        if (sep)
+            HIPCHK(launch_transmission_sep(c->V.p, c->mt_trans.p, nullptr, c->n, c->nz, binning(c), c->p.sigma,
+                                           c->p.potential_pixelsize, r->stream));
+        else
+            HIPCHK(launch_transmission(c->V.p, c->mt_trans.p, c->n, c->nz, binning(c), 0, c->p.sigma,
+                                       c->p.potential_pixelsize, r->stream));
         HIPCHK(launch_plane_fft(c->n, c->probe.p, c->mt_probeF.p, c->tw.p, 0, 1.f, 0.f, r->ws->planes.p, r->stream));
         tab[j] = TiltArgs{c->mt_trans.p, c->mt_probeF.p, c->sqrtM.p, c->x.p, c->y.p, r->err_all.p + off[j]};
         for (int k = 0; k < c->npos; ++k) items[off[j] + k] = (j << ITEM_TILT_SHIFT) | k;   // tilt-major
@@ -1135,6 +1205,7 @@ int mset_recon_error_pass(mset_recon* r, double* errs) {
     A.first = 0;
     A.count = (int)tot;
     A.planes = plane_in_global(r->n) ? r->ws->planes.p : nullptr;
+    A.sep = sep ? 1 : 0;
     {
         Timed t(c0, "position_fwd");
         HIPCHK(launch_fwd_tilts(r->n, A, grid, r->stream));
@@ -1204,6 +1275,18 @@ int mset_recon_set_slot_limit(mset_recon* r, int slots) {
         HIPCHK(hipStreamSynchronize(r->stream));
         drop_graph(r);   // the captured pass holds the old grid
         r->ws->slot_limit = slots;
+    }
+    return MSET_OK;
+}
+
+int mset_recon_set_engine(mset_recon* r, int engine) {
+    if (!r) return fail(MSET_EINVAL, "null reconstruction");
+    if (engine != MSET_ENGINE_AUTO && engine != MSET_ENGINE_GENERAL) return fail(MSET_EINVAL, "bad engine");
+    HIPCHK(hipSetDevice(r->device));
+    if (engine != r->ws->engine) {
+        HIPCHK(hipStreamSynchronize(r->stream));
+        drop_graph(r);   // the captured pass holds the other routine's kernels and layouts
+        r->ws->engine = engine;
     }
     return MSET_OK;
 }

@@ -45,6 +45,43 @@ __global__ void k_trans(const float* __restrict__ V, float2* __restrict__ T, int
     }
 }
 
+// Transmission planes for the separable-propagator kernels: plane a goes to Tf in the layout of the pass that
+// applies it in the forward sweep (rows, [x][y], for even a; columns, [y][x], for odd a) and, if Tb is
+// given, to Tb in the layout of its adjoint pass (the other one).
+__global__ void k_trans_sep(const float* __restrict__ V, float2* __restrict__ Tf, float2* __restrict__ Tb, int n, int nz,
+                            int bin, int L, float sigma, float dx) {
+    const long long n2 = (long long)n * n, tot = n2 * L;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < tot; i += (long long)gridDim.x * blockDim.x) {
+        const int a = (int)(i / n2), p = (int)(i - (long long)a * n2);
+        const int y = p / n, x = p % n;
+        const int z0 = a * bin, z1 = min(z0 + bin, nz);
+        float s = 0.f;
+        for (int z = z0; z < z1; ++z) s += V[(size_t)z * n2 + p];
+        const float ph = sigma * (s * dx);                      // STEP02_FORWARD_4D.m:18-19 rounding order
+        float sn, cs;
+        sincosf(ph, &sn, &cs);
+        const float2 t = make_float2(cs, sn);
+        const long long rowmaj = (long long)y * n + x, colmaj = (long long)x * n + y;
+        Tf[(long long)a * n2 + ((a & 1) ? rowmaj : colmaj)] = t;
+        if (Tb) Tb[(long long)a * n2 + ((a & 1) ? colmaj : rowmaj)] = t;
+    }
+}
+
+// The separable kernels leave odd gradient planes transposed ([x][y]); swap them back to [y][x] in place.
+__global__ void k_untranspose_odd(float* __restrict__ G, int n, int L) {
+    const long long n2 = (long long)n * n, tot = n2 * (L / 2);
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < tot; i += (long long)gridDim.x * blockDim.x) {
+        const int a = 2 * (int)(i / n2) + 1;
+        const int p = (int)(i % n2), y = p / n, x = p % n;
+        if (x < y) {
+            float* P = G + (size_t)a * n2;
+            const float u = P[(size_t)y * n + x], w = P[(size_t)x * n + y];
+            P[(size_t)y * n + x] = w;
+            P[(size_t)x * n + y] = u;
+        }
+    }
+}
+
 __global__ void k_fill(uint4* p, size_t n16) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
         p[i] = make_uint4(0, 0, 0, 0);
@@ -116,6 +153,19 @@ hipError_t launch_transmission(const float* V, float2* T, int n, int nz, int bin
                                hipStream_t st) {
     const int L = (nz + bin - 1) / bin;
     hipLaunchKernelGGL(k_trans, grid_for((size_t)n * n * L), dim3(256), 0, st, V, T, n * n, nz, bin, L, sset, sigma, dx);
+    return hipGetLastError();
+}
+
+hipError_t launch_transmission_sep(const float* V, float2* Tf, float2* Tb, int n, int nz, int bin, float sigma, float dx,
+                                   hipStream_t st) {
+    const int L = (nz + bin - 1) / bin;
+    hipLaunchKernelGGL(k_trans_sep, grid_for((size_t)n * n * L), dim3(256), 0, st, V, Tf, Tb, n, nz, bin, L, sigma, dx);
+    return hipGetLastError();
+}
+
+hipError_t launch_untranspose_odd(float* G, int n, int L, hipStream_t st) {
+    if (L < 2) return hipSuccess;
+    hipLaunchKernelGGL(k_untranspose_odd, grid_for((size_t)n * n * (L / 2)), dim3(256), 0, st, G, n, L);
     return hipGetLastError();
 }
 

@@ -28,6 +28,54 @@
 
 namespace mset {
 
+// Tile steps of a chain pass (TiledEngine::chain): they exchange the whole tile with a global array kept
+// in the OTHER orientation's layout, through the LDS tile, so that access is coalesced as well.
+template <class F> struct IsTileStep { static constexpr bool value = false; };
+
+// stores the tile's current values (unchanged) into dst; no-op when dst is null
+struct TileStore {
+    float2* dst;
+    template <bool COL, class E, int R> __device__ __forceinline__ void run(E& e, float2 (&v)[R], int l0, int line,
+                                                                            int j) const {
+        if (dst) e.tile_to_other(v, dst, l0, line, j);
+    }
+};
+template <> struct IsTileStep<TileStore> { static constexpr bool value = true; };
+
+// loads one operand per element from src and combines it: v = ap(point, v, operand)
+template <class T, class Ap> struct TileLoad {
+    const T* src;
+    Ap ap;
+    template <bool COL, class E, int R> __device__ __forceinline__ void run(E& e, float2 (&v)[R], int l0, int line,
+                                                                            int j) const {
+        T o[R];
+        e.other_to_tile(src, o, l0, line, j);
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = ap(e.template pt<COL>(l0, line, j + R * r), v[r], o[r]);
+    }
+};
+template <class T, class Ap> struct IsTileStep<TileLoad<T, Ap>> { static constexpr bool value = true; };
+template <class T, class Ap> __device__ __forceinline__ TileLoad<T, Ap> tile_load(const T* src, Ap ap) {
+    return TileLoad<T, Ap>{src, ap};
+}
+
+// several steps applied in order as one step of a chain pass
+template <class A, class B, class C = NoOp> struct Seq {
+    A a;
+    B b;
+    C c;
+};
+template <class F> struct IsSeq { static constexpr bool value = false; };
+template <class A, class B, class C> struct IsSeq<Seq<A, B, C>> { static constexpr bool value = true; };
+template <class A, class B> __device__ __forceinline__ Seq<A, B> seq(A a, B b) { return Seq<A, B>{a, b, NoOp()}; }
+template <class A, class B, class C> __device__ __forceinline__ Seq<A, B, C> seq(A a, B b, C c) {
+    return Seq<A, B, C>{a, b, c};
+}
+template <class F> struct HasTileStep { static constexpr bool value = IsTileStep<F>::value; };
+template <class A, class B, class C> struct HasTileStep<Seq<A, B, C>> {
+    static constexpr bool value = HasTileStep<A>::value || HasTileStep<B>::value || HasTileStep<C>::value;
+};
+
 template <int N, int NT_> struct TiledEngine {
     static constexpr int NT = NT_;
     static constexpr int TL = 32;                // lines per tile
@@ -176,6 +224,119 @@ template <int N, int NT_> struct TiledEngine {
         float2* old = RS;
         RS = other;
         return old;
+    }
+
+    // ---------------------------------------------------------------- chain passes
+    // A chain pass runs up to four line transforms D1..D4 on each tile, with a step between consecutive
+    // transforms (and after the last): the tile is loaded once (through `pre`), stored once (SWAP: to the
+    // other buffer through LDS; NONE: not at all), and everything between stays in registers.  Steps are
+    // NoOp, plain element functors, staged functors (their global operands loaded for the whole tile
+    // first) or tile steps (TileLoad / TileStore below), which exchange the tile with a global array in
+    // the OTHER orientation's layout through the LDS tile, so that access stays coalesced too.
+    enum { ST_NONE = 0, ST_SWAP = 1 };
+
+    template <bool COL> __device__ __forceinline__ Pt pt(int l0, int line, int pos) const {
+        const int ln = l0 + line;
+        return COL ? Pt{pos, ln, Idx{(unsigned)(pos * N + ln), 0u}} : Pt{ln, pos, Idx{(unsigned)(pos * N + ln), 0u}};
+    }
+
+    // one step applied to the thread's 16 elements of the tile
+    template <bool COL, class M> __device__ __forceinline__ void step(const M& m, float2 (&v)[R], int l0, int line, int j) {
+        if constexpr (std::is_same<M, NoOp>::value) {
+            return;
+        } else if constexpr (IsSeq<M>::value) {
+            step<COL>(m.a, v, l0, line, j);
+            step<COL>(m.b, v, l0, line, j);
+            step<COL>(m.c, v, l0, line, j);
+        } else if constexpr (IsTileStep<M>::value) {
+            m.template run<COL>(*this, v, l0, line, j);
+        } else if constexpr (IsStaged<M>::value) {
+            decltype(m.load(Pt{})) op[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) op[r] = m.load(pt<COL>(l0, line, j + R * r));
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[r] = m.apply(pt<COL>(l0, line, j + R * r), v[r], op[r]);
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[r] = m(pt<COL>(l0, line, j + R * r), v[r]);
+        }
+    }
+
+    template <bool COL, int ST, int D1, int D2 = NONE, int D3 = NONE, int D4 = NONE, class P0 = NoOp, class M1 = NoOp,
+              class M2 = NoOp, class M3 = NoOp, class M4 = NoOp>
+    __device__ __forceinline__ void chain(const P0& pre, const M1& m1 = M1(), const M2& m2 = M2(), const M3& m3 = M3(),
+                                          const M4& m4 = M4()) {
+        const int t = threadIdx.x;
+        const int line = t % TL, j = t / TL;
+        const float2* in = COL ? RS : XS;
+        float2* out = COL ? XS : RS;
+        constexpr bool PRE_TILE = IsTileStep<P0>::value;   // a tile load that supplies the values
+        for (int l0 = 0; l0 < N; l0 += TL) {
+            float2 v[R];
+            if constexpr (PRE_TILE) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[r] = make_float2(0.f, 0.f);
+                pre.template run<COL>(*this, v, l0, line, j);
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[r] = in[(j + R * r) * N + l0 + line];
+                step<COL>(pre, v, l0, line, j);
+            }
+            if constexpr (D1 != NONE) dft<D1, PRE_TILE>(v, line, j);
+            step<COL>(m1, v, l0, line, j);
+            if constexpr (D2 != NONE) dft<D2, true>(v, line, j);
+            step<COL>(m2, v, l0, line, j);
+            if constexpr (D3 != NONE) dft<D3, true>(v, line, j);
+            step<COL>(m3, v, l0, line, j);
+            if constexpr (D4 != NONE) dft<D4, true>(v, line, j);
+            step<COL>(m4, v, l0, line, j);
+            if constexpr (ST == ST_SWAP) {
+                // after a transform each thread rewrites the LDS words it read last; after a tile step the
+                // words differ, so the tile step's readers must be done first
+                constexpr bool T1 = HasTileStep<M1>::value, T2 = HasTileStep<M2>::value;
+                constexpr bool T3 = HasTileStep<M3>::value, T4 = HasTileStep<M4>::value;
+                constexpr bool TILE_LAST = (D4 != NONE) ? T4
+                                           : (D3 != NONE) ? (T3 || T4)
+                                           : (D2 != NONE) ? (T2 || T3 || T4)
+                                           : (D1 != NONE) ? (T1 || T2 || T3 || T4)
+                                           : true;
+                if constexpr (TILE_LAST) __syncthreads();
+#pragma unroll
+                for (int r = 0; r < R; ++r) s[(j + R * r) * TP + line] = v[r];
+                __syncthreads();
+                for (int e = t; e < TL * N; e += NT) {
+                    const int pos = e % N, l = e / N;
+                    out[(l0 + l) * N + pos] = s[pos * TP + l];
+                }
+            }
+            __syncthreads();
+        }
+    }
+
+    // LDS round trip of a tile between the thread layout (16 elements of one line) and a global array in
+    // the other orientation's layout (lines slow, positions fast): element (line l0 + l, position pos) of
+    // the array is dst[(l0 + l) N + pos].  Callers ensure the LDS tile is free (a barrier, or a transform's
+    // LEAD barrier after).
+    __device__ __forceinline__ void tile_to_other(const float2 (&v)[R], float2* dst, int l0, int line, int j) {
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; ++r) s[(j + R * r) * TP + line] = v[r];
+        __syncthreads();
+        for (int e = threadIdx.x; e < TL * N; e += NT) {
+            const int pos = e % N, l = e / N;
+            dst[(l0 + l) * N + pos] = s[pos * TP + l];
+        }
+    }
+    template <class T> __device__ __forceinline__ void other_to_tile(const T* src, T (&o)[R], int l0, int line, int j) {
+        __syncthreads();
+        T* st = reinterpret_cast<T*>(s);   // T is float or float2: at most one float2 per LDS slot
+        for (int e = threadIdx.x; e < TL * N; e += NT) {
+            const int pos = e % N, l = e / N;
+            st[(pos * TP + l) * (sizeof(float2) / sizeof(T))] = src[(l0 + l) * N + pos];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; ++r) o[r] = st[((j + R * r) * TP + line) * (sizeof(float2) / sizeof(T))];
     }
 
     template <class F> __device__ __forceinline__ void each_R(const F& f) {

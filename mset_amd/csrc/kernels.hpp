@@ -49,6 +49,10 @@ struct PosArgs {
     float* gslab;                // object-gradient slabs, L planes each (K_PLAIN: slab 0, this position)
     float2* pslab;               // K_BATCH: probe-gradient slabs, one plane each (Fourier domain)
     int nslab;                   // K_BATCH: position p of the launch adds to slab p % nslab
+    int sep;                     // N = 256: separable-propagator routine (p1d, trans in the forward layouts,
+                                 // trans_b in the adjoint layouts; odd gradient planes transposed)
+    const float2* p1d;           // sep: the propagator's 1D factors px[N], py[N] (p(kx, ky) = px(kx) py(ky))
+    const float2* trans_b;       // sep, K_BATCH: transmission planes in the adjoint passes' layouts
     float2* planes;              // N = 256: per-slot plane buffers (2 N^2), the tiled engine's state
     float* dpos;                 // K_BATCH: per-position update terms (2 per position)
 };
@@ -73,6 +77,11 @@ hipError_t launch_ingest_patterns(const float* M, float* sqrtM, int n, long long
 hipError_t launch_transmission(const float* V, float2* trans, int n, int nz, int bin, int sset, float sigma, float dx,
                                hipStream_t st);
 hipError_t launch_fill(void* p, size_t bytes, hipStream_t st);
+// separable-propagator kernels: transmission planes in the forward (Tf) and adjoint (Tb, nullable) layouts,
+// and the in-place fix of the odd gradient planes they leave transposed
+hipError_t launch_transmission_sep(const float* V, float2* Tf, float2* Tb, int n, int nz, int bin, float sigma, float dx,
+                                   hipStream_t st);
+hipError_t launch_untranspose_odd(float* G, int n, int L, hipStream_t st);
 hipError_t launch_reduce_slabs(const float* slab, int nslab, size_t n, float* out, hipStream_t st);
 hipError_t launch_reduce_cslabs(const float2* slab, int nslab, size_t n, float2* out, hipStream_t st);
 hipError_t launch_update_volume(float* V, const float* G, int n, int nz, int bin, float s1, hipStream_t st);

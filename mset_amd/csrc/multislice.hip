@@ -404,6 +404,209 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
     }
 }
 
+// ---------------------------------------------------------------------------------------------------
+// Separable propagator, planes in global memory (the 256 x 256 engine).
+//
+// The Fresnel propagator of Func_generate_propagator.m, exp(-i pi lambda dz (qx^2 + qy^2)), is a product
+// p(kx, ky) = px(kx) py(ky), so one propagation step is P = Px Py with Px = Fx^-1 px Fx acting along x
+// alone and Py along y alone; they commute.  Alternating their order from plane to plane,
+//     t0, Px Py, t1, Py Px, t2, Px Py, ...
+// puts two half-steps along the same axis around every transmission multiply, so each chain pass runs
+// [F, x p, F^-1, (store psi_a, x t_a), F, x p, F^-1] along one axis: a plane costs one pass over the
+// global plane buffers instead of two.  The adjoint does the same with conj(p) and the gradient step.
+// The 1D factors live in LDS, so the propagator costs no global loads.
+//
+// Orientation of pass X_k: columns (along y) for even k, rows (along x) for odd k.  The forward step of
+// plane a is in X_(a+1), the adjoint step in X_(2L+2-a): the reference pairing conj(psi_(a+1)) reads each
+// stored wave in the orientation that wrote it, while the transmission of plane a is read in opposite
+// orientations, so it is kept in both layouts (trans: forward, trans_b: adjoint).  Gradient slab planes
+// are in their adjoint pass's layout (odd planes transposed; the host puts them back after the sum).
+// Arrays kept in the other layout than the pass that reads them (fftn(probe) and sqrt(M), [x][y], in a
+// column pass; psi0, [y][x], written in a row pass) go through the LDS tile (TileLoad / TileStore).
+//
+// Used for the batched gradient with the reference pairing and stored waves, and for the forward-only
+// kernels; the sequential kernel, the exact pairing and the recomputed waves use process_position.
+template <int N, int NT, int KIND, class Eng>
+__device__ __forceinline__ void process_sep(const PosArgs& A, int k, int slot, int slab, bool first, Eng& E,
+                                            float2* phx, float2* phy, const float2* px, const float2* py,
+                                            double* red) {
+    static_assert(KIND == K_BATCH || KIND == K_FWD, "separable path: batched gradient or forward only");
+    using CT = std::true_type;    // column pass
+    using RT = std::false_type;   // row pass
+    constexpr int N2 = N * N;
+    constexpr int FWD = Eng::FWD, INV = Eng::INV, SWAP = Eng::ST_SWAP, NOST = Eng::ST_NONE;
+    const int t = threadIdx.x;
+    const int L = A.L;
+    const float inv_n = 1.0f / (float)N, inv_n2 = 1.0f / (float)N2;
+    constexpr bool GRAD = KIND == K_BATCH;
+    float2* save = A.save + (size_t)slot * (L + 2) * N2;
+    E.bind(A.planes + (size_t)slot * 2 * N2);
+    const bool want_pos = GRAD && (A.want & WANT_POS) != 0;
+    const bool want_probe = GRAD && (A.want & WANT_PROBE) != 0;
+
+    // ---- sub-pixel shift factors, separable (Func_generate_probe_wave_scan_sub_pixel.m:4-27)
+    const double row = (double)A.xlist[k], col = (double)A.ylist[k];
+    const double dxs = row - (double)A.cent, dys = col - (double)A.cent;
+    for (int j = t; j < 2 * N; j += NT) {
+        const int jj = j < N ? j : j - N;
+        const double d = j < N ? dxs : dys;
+        const double a = -2.0 * M_PI * d * shift_grid<N>(jj) / (double)N;
+        double sn, cs;
+        sincos(a, &sn, &cs);
+        (j < N ? phx : phy)[jj] = make_float2((float)cs, (float)sn);
+    }
+    __syncthreads();
+    auto phase = [&](int y, int x) { return cmul(phy[y], phx[x]); };
+
+    // propagation factors along the pass axis, with the 1/N of that axis' inverse transform
+    auto pfac = [&](auto colt) {
+        return [&](const Pt& p, float2 v) {
+            return cscale(cmul(v, decltype(colt)::value ? py[p.y] : px[p.x]), inv_n);
+        };
+    };
+    auto bfac = [&](auto colt) {
+        return [&](const Pt& p, float2 v) {
+            return cscale(cmulc(v, decltype(colt)::value ? py[p.y] : px[p.x]), inv_n);
+        };
+    };
+
+    // ---- X0 (columns): fftn(probe) .* Phi, inverse along y.  fftn(probe) is [x][y]: through the LDS tile
+    E.template chain<true, SWAP, INV>(tile_load(A.probeF, [&](const Pt& p, float2, float2 f) {
+        return cmul(f, phase(p.y, p.x));
+    }));
+    // ---- X1 (rows): inverse along x -> psi0 (stored [y][x] through the LDS tile), x t_0, half-step along x
+    {
+        const BufC t0(A.trans, N2);
+        auto scale = [&](const Pt&, float2 v) { return cscale(v, inv_n2); };
+        auto times_t = staged<16>([&](const Pt& p) { return t0[p.e]; }, [&](const Pt&, float2 v, float2 o) { return cmul(v, o); });
+        E.template chain<false, SWAP, INV, FWD, INV>(NoOp(), seq(scale, TileStore{GRAD ? save : nullptr}, times_t),
+                                                     pfac(RT{}));
+    }
+    // ---- X_(a+1), a = 1 .. L-1: finish plane a-1's step, store psi_a, x t_a, start plane a's step
+    auto fwd_pass = [&](auto colt, int a) {
+        constexpr bool COL = decltype(colt)::value;
+        const BufC ta(A.trans + (size_t)a * N2, N2);
+        const BufC sv(save + (size_t)a * N2, N2);
+        auto mid = staged<16>([&](const Pt& p) { return ta[p.e]; }, [&](const Pt& p, float2 v, float2 o) {
+            if (GRAD) sv.put(p.e, v);
+            return cmul(v, o);
+        });
+        E.template chain<COL, SWAP, FWD, INV, FWD, INV>(NoOp(), pfac(colt), mid, pfac(colt));
+    };
+    for (int a = 1; a < L; ++a) {
+        if ((a + 1) % 2 == 0) fwd_pass(CT{}, a);
+        else fwd_pass(RT{}, a);
+    }
+    // ---- X_(L+1): finish plane L-1's step, store psi_L, forward transform along this axis
+    auto last_pass = [&](auto colt) {
+        constexpr bool COL = decltype(colt)::value;
+        const BufC sv(save + (size_t)L * N2, N2);
+        auto keep = [&](const Pt& p, float2 v) {
+            if (GRAD) sv.put(p.e, v);
+            return v;
+        };
+        E.template chain<COL, SWAP, FWD, INV, FWD>(NoOp(), pfac(colt), keep);
+    };
+    if ((L + 1) % 2 == 0) last_pass(CT{});
+    else last_pass(RT{});
+
+    // ---- X_(L+2): forward along the other axis -> exit wave Psi; intensity, error, residual
+    // (STEP02_FORWARD_4D.m:36-49); the residual times conj(p) and the inverse along this axis start the adjoint
+    double err = 0.0;
+    {
+        float* Iout = A.inten ? A.inten + (size_t)k * N2 : nullptr;
+        const float* MT = A.sqrtM + (size_t)k * N2;                     // sqrt(M), [x][y]
+        auto resid = [&](const Pt& p, float2 v, float m) {
+            const float I = v.x * v.x + v.y * v.y;
+            if (KIND == K_FWD && Iout) Iout[((p.y + N / 2) % N) * N + (p.x + N / 2) % N] = I;
+            const float d = m - __builtin_amdgcn_sqrtf(I);
+            err += (double)(d * d);
+            if (KIND == K_FWD) return v;
+            const float f = 1.0f - m * __builtin_amdgcn_rsqf(I + 1e-30f);
+            return cmulc(make_float2(v.x * f, v.y * f), cmul(px[p.x], py[p.y]));
+        };
+        if (L % 2 == 0) {
+            // column pass: sqrt(M) is [x][y], through the LDS tile
+            auto ex = tile_load(MT, resid);
+            if constexpr (GRAD) E.template chain<true, SWAP, FWD, INV>(NoOp(), ex);
+            else E.template chain<true, NOST, FWD>(NoOp(), ex);
+        } else {
+            const BufF MB(MT, N2);
+            auto ex = staged<16>([&](const Pt& p) { return MB[p.e]; }, resid);
+            if constexpr (GRAD) E.template chain<false, SWAP, FWD, INV>(NoOp(), ex);
+            else E.template chain<false, NOST, FWD>(NoOp(), ex);
+        }
+    }
+    err = block_sum<NT>(err, red);
+    if (t == 0) A.err_pos[k - A.obase] = err;
+    if constexpr (KIND == K_FWD) return;
+
+    // ---- adjoint (STEP03_BACK.m:12-24): plane a's step in X_(2L+2-a), columns for even a
+    double dr = 0.0, dc = 0.0;
+    auto adj_pass = [&](auto colt, int a) {
+        constexpr bool COL = decltype(colt)::value;
+        const BufC tb(A.trans_b + (size_t)a * N2, N2);
+        const BufC ws(save + (size_t)(a + 1) * N2, N2);              // reference pairing: the wave after plane a
+        const BufF g(A.gslab + ((size_t)slab * L + a) * N2, N2);
+        const BufF gin(A.gslab + ((size_t)slab * L + a) * N2, first ? 0u : N2);   // empty view: reads 0
+        auto mid = staged<16>([&](const Pt& p) { return AdjOp{tb[p.e], ws[p.e], gin[p.e]}; },
+                              [&](const Pt& p, float2 u, const AdjOp& o) {
+                                  const float2 tc = cconj(o.t);
+                                  const float2 q = cmul(tc, cmulc(u, o.w));
+                                  g.put(p.e, o.g - q.y);                // Re(1i * conj(t) * conj(psi) * b)
+                                  return cmul(u, tc);
+                              });
+        auto scale = [&](const Pt&, float2 v) { return cscale(v, inv_n2); };
+        const bool head = a == L - 1, tail = a == 0;
+        if constexpr (COL) {
+            // plane 0 is a column pass: the position terms (STEP03_BACK.m:35-38) ride on it, from bw =
+            // conj(t_0) .* b and MATLAB gradient() of conj(psi0) (psi0 is [y][x]: neighbours are +-1, +-N)
+            const float2* p0 = save;
+            auto pos = [&](const Pt& p, float2 bw) {
+                if (want_pos) {
+                    const int y = p.y, x = p.x;
+                    const int xp = min(x + 1, N - 1), xm = max(x - 1, 0), yp = min(y + 1, N - 1), ym = max(y - 1, 0);
+                    const float2 gx = cscale(csub(p0[y * N + xp], p0[y * N + xm]), xp - xm == 2 ? 0.5f : 1.0f);
+                    const float2 gy = cscale(csub(p0[yp * N + x], p0[ym * N + x]), yp - ym == 2 ? 0.5f : 1.0f);
+                    dr += (double)(gx.x * bw.x + gx.y * bw.y);
+                    dc += (double)(gy.x * bw.x + gy.y * bw.y);
+                }
+                return bw;
+            };
+            if (head && tail) E.template chain<true, SWAP, INV, FWD>(NoOp(), seq(scale, mid, pos));
+            else if (head) E.template chain<true, SWAP, INV, FWD, INV>(NoOp(), seq(scale, mid), bfac(colt));
+            else if (tail) E.template chain<true, SWAP, FWD, INV, FWD>(NoOp(), bfac(colt), seq(mid, pos));
+            else E.template chain<true, SWAP, FWD, INV, FWD, INV>(NoOp(), bfac(colt), mid, bfac(colt));
+        } else {
+            if (head) E.template chain<false, SWAP, INV, FWD, INV>(NoOp(), seq(scale, mid), bfac(colt));
+            else E.template chain<false, SWAP, FWD, INV, FWD, INV>(NoOp(), bfac(colt), mid, bfac(colt));
+        }
+    };
+    for (int a = L - 1; a >= 0; --a) {
+        if (a % 2 == 0) adj_pass(CT{}, a);
+        else adj_pass(RT{}, a);
+    }
+    if (want_pos) {
+        dr = block_sum<NT>(dr, red);
+        dc = block_sum<NT>(dc, red);
+        if (t == 0) {
+            A.dpos[2 * (k - A.obase)] = (float)dr;
+            A.dpos[2 * (k - A.obase) + 1] = (float)dc;
+        }
+    }
+    // ---- X_(2L+3) (rows): forward along x -> fftn(bw); the probe term in the Fourier domain,
+    // shift(bw, -d) = ifftn(fftn(bw) .* conj(Phi)), added to the slab ([x][y])
+    if (want_probe) {
+        const BufC ps(A.pslab + (size_t)slab * N2, N2);
+        const BufC psin(A.pslab + (size_t)slab * N2, first ? 0u : N2);   // empty view: reads 0
+        auto term = staged<16>([&](const Pt& p) { return psin[p.e]; }, [&](const Pt& p, float2 v, float2 o) {
+            ps.put(p.e, cadd(o, cmulc(v, phase(p.y, p.x))));
+            return v;
+        });
+        E.template chain<false, NOST, FWD>(NoOp(), term);
+    }
+}
+
 template <int N, int NT> __device__ __forceinline__ void load_twiddles(float2* tw, const float2* g) {
     for (int i = threadIdx.x; i < N; i += NT) tw[i] = g[i];
     __syncthreads();
@@ -429,24 +632,45 @@ __global__ __launch_bounds__(Cfg<N>::NT, Cfg<N>::MINB) void k_plain(PosArgs A, i
 // none of its code.  Position p of the launch adds to slab p % nslab.  The host launches either at most
 // nslab positions or a grid that divides nslab, so every slab's positions run on one workgroup in
 // increasing p: each slab sums the same positions in the same order whatever the grid size.
-template <int N, bool RC>
+// SEP: the separable-propagator routine (process_sep; the 256 x 256 engine, stored waves), its 1D factors in LDS.
+template <int N, bool RC, bool SEP>
 __global__ __launch_bounds__(Cfg<N>::NT, Cfg<N>::MINB) void k_batch(PosArgs A) {
     constexpr int NT = Cfg<N>::NT;
     MSET_LDS(N, NT)
-    for (int p = blockIdx.x; p < A.count; p += gridDim.x) {
-        const int k = A.order ? A.order[A.first + p] : A.first + p;
-        const int slab = p % A.nslab;
-        process_position<N, NT, K_BATCH, RC>(A, k, blockIdx.x, slab, p < A.nslab, E, phx, phy, red);
+    if constexpr (SEP) {
+        __shared__ float2 px[N], py[N];
+        for (int i = threadIdx.x; i < 2 * N; i += NT) (i < N ? px : py)[i % N] = A.p1d[i];
         __syncthreads();
+        for (int p = blockIdx.x; p < A.count; p += gridDim.x) {
+            const int k = A.order ? A.order[A.first + p] : A.first + p;
+            process_sep<N, NT, K_BATCH>(A, k, blockIdx.x, p % A.nslab, p < A.nslab, E, phx, phy, px, py, red);
+            __syncthreads();
+        }
+    } else {
+        for (int p = blockIdx.x; p < A.count; p += gridDim.x) {
+            const int k = A.order ? A.order[A.first + p] : A.first + p;
+            const int slab = p % A.nslab;
+            process_position<N, NT, K_BATCH, RC>(A, k, blockIdx.x, slab, p < A.nslab, E, phx, phy, red);
+            __syncthreads();
+        }
     }
 }
 
 // MT: the work items are positions of several tilts (A.tilts, A.order), so a whole error pass over a tilt
 // series is one persistent launch; a separate instantiation, so the one-tilt kernel carries none of it
-template <int N, bool MT>
+template <int N, bool MT, bool SEP>
 __global__ __launch_bounds__(Cfg<N>::NT, Cfg<N>::MINB) void k_fwd(PosArgs A) {
     constexpr int NT = Cfg<N>::NT;
     MSET_LDS(N, NT)
+    float2* px = nullptr;
+    float2* py = nullptr;
+    if constexpr (SEP) {
+        __shared__ float2 pxs[N], pys[N];
+        for (int i = threadIdx.x; i < 2 * N; i += NT) (i < N ? pxs : pys)[i % N] = A.p1d[i];
+        __syncthreads();
+        px = pxs;
+        py = pys;
+    }
     for (int p = blockIdx.x; p < A.count; p += gridDim.x) {
         if constexpr (MT) {
             const int item = A.order[p];
@@ -459,10 +683,12 @@ __global__ __launch_bounds__(Cfg<N>::NT, Cfg<N>::MINB) void k_fwd(PosArgs A) {
             B.ylist = T.ylist;
             B.err_pos = T.err_pos;
             B.obase = 0;
-            process_position<N, NT, K_FWD>(B, item & ITEM_POS_MASK, blockIdx.x, blockIdx.x, true, E, phx, phy, red);
+            if constexpr (SEP) process_sep<N, NT, K_FWD>(B, item & ITEM_POS_MASK, blockIdx.x, blockIdx.x, true, E, phx, phy, px, py, red);
+            else process_position<N, NT, K_FWD>(B, item & ITEM_POS_MASK, blockIdx.x, blockIdx.x, true, E, phx, phy, red);
         } else {
             const int k = A.order ? A.order[A.first + p] : A.first + p;
-            process_position<N, NT, K_FWD>(A, k, blockIdx.x, blockIdx.x, p == (int)blockIdx.x, E, phx, phy, red);
+            if constexpr (SEP) process_sep<N, NT, K_FWD>(A, k, blockIdx.x, blockIdx.x, true, E, phx, phy, px, py, red);
+            else process_position<N, NT, K_FWD>(A, k, blockIdx.x, blockIdx.x, p == (int)blockIdx.x, E, phx, phy, red);
         }
         __syncthreads();
     }
@@ -541,18 +767,27 @@ hipError_t launch_plain(int n, const PosArgs& A, int k, hipStream_t st) {
 }
 
 hipError_t launch_batch(int n, const PosArgs& A, int grid, hipStream_t st) {
-    if (A.recompute) MSET_DISPATCH(n, hipLaunchKernelGGL((k_batch<NN, true>), dim3(grid), dim3(Cfg<NN>::NT), 0, st, A))
-    else MSET_DISPATCH(n, hipLaunchKernelGGL((k_batch<NN, false>), dim3(grid), dim3(Cfg<NN>::NT), 0, st, A));
+    if (A.sep) {
+        if (n != 256 || A.recompute || A.grad_mode != GRAD_REF || A.sset) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_batch<256, false, true>), dim3(grid), dim3(Cfg<256>::NT), 0, st, A);
+    } else if (A.recompute) MSET_DISPATCH(n, hipLaunchKernelGGL((k_batch<NN, true, false>), dim3(grid), dim3(Cfg<NN>::NT), 0, st, A))
+    else MSET_DISPATCH(n, hipLaunchKernelGGL((k_batch<NN, false, false>), dim3(grid), dim3(Cfg<NN>::NT), 0, st, A));
     return hipGetLastError();
 }
 
 hipError_t launch_fwd(int n, const PosArgs& A, int grid, hipStream_t st) {
-    MSET_DISPATCH(n, hipLaunchKernelGGL((k_fwd<NN, false>), dim3(grid), dim3(Cfg<NN>::NT), 0, st, A));
+    if (A.sep) {
+        if (n != 256 || A.sset) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_fwd<256, false, true>), dim3(grid), dim3(Cfg<256>::NT), 0, st, A);
+    } else MSET_DISPATCH(n, hipLaunchKernelGGL((k_fwd<NN, false, false>), dim3(grid), dim3(Cfg<NN>::NT), 0, st, A));
     return hipGetLastError();
 }
 
 hipError_t launch_fwd_tilts(int n, const PosArgs& A, int grid, hipStream_t st) {
-    MSET_DISPATCH(n, hipLaunchKernelGGL((k_fwd<NN, true>), dim3(grid), dim3(Cfg<NN>::NT), 0, st, A));
+    if (A.sep) {
+        if (n != 256 || A.sset) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_fwd<256, true, true>), dim3(grid), dim3(Cfg<256>::NT), 0, st, A);
+    } else MSET_DISPATCH(n, hipLaunchKernelGGL((k_fwd<NN, true, false>), dim3(grid), dim3(Cfg<NN>::NT), 0, st, A));
     return hipGetLastError();
 }
 
@@ -564,7 +799,7 @@ hipError_t launch_plane_fft(int n, const float2* in, float2* out, const float2* 
 }
 
 hipError_t occupancy_batch(int n, int* blocks_per_cu) {
-    MSET_DISPATCH(n, return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batch<NN, false>, Cfg<NN>::NT, 0));
+    MSET_DISPATCH(n, return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, (k_batch<NN, false, false>), Cfg<NN>::NT, 0));
     return hipSuccess;
 }
 

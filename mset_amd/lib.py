@@ -27,9 +27,11 @@ EXPORTS = ['mset_create', 'mset_destroy', 'mset_last_error', 'mset_stream', 'mse
            'mset_recon_snapshot', 'mset_recon_restore', 'mset_recon_profile_enable', 'mset_recon_profile_read',
            'mset_recon_stream', 'mset_tv', 'mset_recon_tv', 'mset_device_memory', 'mset_batch_slots',
            'mset_set_exit_waves', 'mset_exit_waves', 'mset_recon_set_exit_waves', 'mset_set_slot_limit',
-           'mset_recon_set_slot_limit', 'mset_host_alloc', 'mset_host_free', 'mset_grad_noalloc']
+           'mset_recon_set_slot_limit', 'mset_host_alloc', 'mset_host_free', 'mset_grad_noalloc', 'mset_set_engine',
+           'mset_recon_set_engine']
 PATH_PLAIN, PATH_BATCHED = 0, 1
 EXIT_WAVES_AUTO, EXIT_WAVES_STORE, EXIT_WAVES_RECOMPUTE = 0, 1, 2
+ENGINE_AUTO, ENGINE_GENERAL = 0, 1
 
 
 class MsetError(RuntimeError):
@@ -85,6 +87,7 @@ def load():
         'mset_recon_set_slot_limit': ([P, I], I), 'mset_host_alloc': ([C.c_size_t, PP], I),
         'mset_host_free': ([P], I),
         'mset_grad_noalloc': ([P, I, I, P, P, P, P], I),
+        'mset_set_engine': ([P, I], I), 'mset_recon_set_engine': ([P, I], I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -310,6 +313,10 @@ class Context:
         """cap on the batched kernels' persistent grid (0 = all resident workgroups)"""
         _chk(self._L.mset_set_slot_limit(self._h, int(slots)))
 
+    def set_engine(self, engine):
+        """ENGINE_AUTO (separable routine at 256 when the propagator factors) or ENGINE_GENERAL"""
+        _chk(self._L.mset_set_engine(self._h, int(engine)))
+
     def exit_waves(self):
         """(variant, per-slot scratch bytes) of the last batched call."""
         m = C.c_int(0)
@@ -430,6 +437,10 @@ class Recon:
     def set_slot_limit(self, slots):
         """cap on the batched passes' persistent grid (0 = all resident workgroups)"""
         _chk(self._L.mset_recon_set_slot_limit(self._h, int(slots)))
+
+    def set_engine(self, engine):
+        """ENGINE_AUTO or ENGINE_GENERAL for the series' passes"""
+        _chk(self._L.mset_recon_set_engine(self._h, int(engine)))
 
     def snapshot(self):
         _chk(self._L.mset_recon_snapshot(self._h))

@@ -9,16 +9,22 @@ from mset_amd import autotune
 class FakeRecon:
     shape = (8, 64, 64)
 
-    def __init__(self, cost):
-        self.cost = cost                # (slots, batch) -> seconds per pass
+    def __init__(self, cost, shape=(8, 64, 64)):
+        self.cost = cost                # (slots, batch) or (engine, slots, batch) -> seconds per pass
+        self.shape = shape
         self.npos = [24, 24]
         self.slots = 0
+        self.engine = 0
         self.calls = []
         self.dirty = False
 
     def set_slot_limit(self, s):
         self.slots = s
         self.calls.append(('slots', s))
+
+    def set_engine(self, e):
+        self.engine = e
+        self.calls.append(('engine', e))
 
     def snapshot(self):
         self.calls.append(('snapshot',))
@@ -29,7 +35,7 @@ class FakeRecon:
 
     def iterate(self, path, batch, use_graph=False, want_error=True):
         assert not use_graph
-        time.sleep(self.cost.get((self.slots, batch), 0.0))
+        time.sleep(self.cost.get((self.engine, self.slots, batch), self.cost.get((self.slots, batch), 0.0)))
         self.dirty = True
         self.calls.append(('iterate', self.slots, batch))
 
@@ -65,3 +71,13 @@ def test_tuner_times_every_candidate_restores_and_caches(tmp_path):
     with open(path) as fh:
         disk = json.load(fh)
     assert [(v['slots'], v['batch']) for v in disk.values()] == [(12, 24)]
+
+
+def test_tuner_sweeps_the_plane_routine_at_256():
+    """At 256 x 256 both plane routines (separable and general) are candidates; the faster one is left set."""
+    from mset_amd import lib
+    cost = {(lib.ENGINE_AUTO, 24, 8): 0.03, (lib.ENGINE_GENERAL, 24, 8): 0.01}
+    r = FakeRecon(cost, shape=(16, 256, 256))
+    best = autotune.tune_recon(r, batches=(8,), slots=[24])
+    assert len(best['table']) == 2 and best['engine'] == lib.ENGINE_GENERAL and r.engine == lib.ENGINE_GENERAL
+    assert autotune.engine_candidates(128) == (lib.ENGINE_AUTO,)

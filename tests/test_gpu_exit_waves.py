@@ -48,7 +48,9 @@ def test_recomputed_waves_give_the_stored_gradient(n, nz, binning, mode):
         ctx.close()
     assert (ms, mr) == (lib.EXIT_WAVES_STORE, lib.EXIT_WAVES_RECOMPUTE)
     assert bs == n * n * ((L + 2) * 8 + L * 4) and br == n * n * (5 * 8 + L * 4)
-    assert abs(rerr / err - 1) < 1e-6                      # same forward pass
+    # the same forward pass, except at 256 with the reference pairing, where the stored-wave variant runs the
+    # separable-propagator routine and the recomputing one the general routine (rounding differs slightly)
+    assert abs(rerr / err - 1) < (1e-5 if n == 256 and mode == O.MODE_REF else 1e-6)
     print('rebuilt vs stored: dV %.2e dprobe %.2e' % (rel(rV, dV), rel(rp, dp)), flush=True)
     assert rel(rV, dV) < REBUILD_TOL
     assert rel(rp, dp) < REBUILD_TOL

@@ -314,3 +314,27 @@ def test_diffraction_transpose_at_ingest(mset):
     mset.upload(c['V'], c['probe'], c['prop'], c['xl'], c['yl'], c['M'], c['sig'], c['dx'], slice_binning=c['bin'],
                 n_scan=(c['npos'], 1))
     assert not np.array_equal(mset.grad()[0], ref[0])
+
+
+@pytest.mark.parametrize('mode', [O.MODE_REF, O.MODE_EXACT])
+def test_256_separable_and_general_propagator_paths_vs_oracle(mset, mode):
+    """At 256 x 256 the Fresnel propagator is separable and the reference-pairing gradient and the forward
+    kernels take the separable routine (one pass per plane); a unit-modulus propagator that does not factor
+    sends the same work through the general routine.  Both against the FP64 oracle with their own
+    propagator."""
+    c = small_case(n=256, nz=10, binning=2, npos=6, seed=77, spread=24.0)
+    n = 256
+    i = np.arange(n)
+    bump = np.exp(0.3j * np.sin(2 * np.pi * np.outer(i, i) / n ** 2 * 7.0))     # does not factor in kx, ky
+    for name, prop in (('separable', c['prop']), ('general', c['prop'] * bump)):
+        bprop = np.conj(prop)
+        print('case: 256 %s propagator, mode %d' % (name, mode), flush=True)
+        mset.upload(c['V'], c['probe'], prop, c['xl'], c['yl'], c['M'], c['sig'], c['dx'], slice_binning=c['bin'],
+                    n_scan=(c['npos'], 1), grad_mode=mode)
+        dV, dp, dpos, err = mset.grad()
+        e1 = mset.error()
+        odV, odp, odpos, oep = O.grad_fixed(c['V'], c['probe'], prop, bprop, c['xl'], c['yl'], c['M'], c['bin'],
+                                            c['sig'], c['dx'], mode=mode)
+        oerr = oep.sum() / c['npos']
+        assert abs(err / oerr - 1) < LOSS_TOL and abs(e1 / oerr - 1) < LOSS_TOL
+        assert rel(dV, odV) < GRAD_TOL and rel(dp, odp) < GRAD_TOL and rel(dpos, odpos) < GRAD_TOL

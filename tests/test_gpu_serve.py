@@ -45,3 +45,54 @@ def test_results_are_the_same_alone_and_in_the_mix():
                     assert np.array_equal(x, y)
             else:
                 assert np.array_equal(a, b)
+
+
+def test_c5_job_draw_alone_and_in_the_mix():
+    """The 16-job serving draw (volumes 32..128^3, probes 64..256^2, 8..30 tilts, 16^2..40^2 scans) with 24
+    requests arriving as a Poisson stream at 20/s, under a 60 GB budget (about three of the largest jobs):
+    every job's errors and final state are the same, bit for bit, when its requests run alone and when they
+    run in the mix, where requests overlap on the device and jobs are evicted and re-admitted."""
+    import time
+    from mset_amd import serve
+    specs = serve.draw_jobs(16, 5)
+    offsets, jobs = serve.poisson_arrivals(24, 20.0, 16, 6)
+    count = np.bincount(jobs, minlength=len(specs))
+    alone_err, alone_state = {}, {}
+    for s in specs:
+        if count[s.job_id] == 0:
+            continue
+        print('case: C5 job %d alone (P=%d, %d tilts, scan %d, %d requests)' % (s.job_id, s.P, len(s.tilts), s.scan,
+                                                                               count[s.job_id]), flush=True)
+        srv = serve.Server([s], workers=1, batch=512)
+        reqs = [srv.submit(s.job_id) for _ in range(count[s.job_id])]
+        for r in reqs:
+            assert r.done.wait(300), 'request did not finish'
+        assert all(not r.failure for r in reqs), [r.failure for r in reqs]
+        alone_err[s.job_id] = list(srv.jobs[s.job_id].errors)
+        alone_state[s.job_id] = srv.state_of(s.job_id)
+        srv.close()
+    print('case: C5 mix, 24 requests, 4 workers, 60 GB budget', flush=True)
+    srv = serve.Server(specs, workers=4, budget_bytes=int(60e9), batch=512)
+    t0 = time.perf_counter()
+    reqs = []
+    for off, j in zip(offsets, jobs):
+        now = time.perf_counter() - t0
+        if off > now:
+            time.sleep(off - now)
+        reqs.append(srv.submit(int(j)))
+    for r in reqs:
+        assert r.done.wait(600), 'request did not finish'
+    assert all(not r.failure for r in reqs), [r.failure for r in reqs]
+    mix_err = {s.job_id: list(srv.jobs[s.job_id].errors) for s in specs}
+    mix_state = {s.job_id: srv.state_of(s.job_id) for s in specs if count[s.job_id]}
+    rep = srv.report()
+    srv.close()
+    print('C5 mix report', {k: rep[k] for k in ('requests', 'p50_ms', 'p99_ms', 'throughput', 'evictions',
+                                               'admissions', 'overlap_s', 'max_concurrent')}, flush=True)
+    assert rep['evictions'] >= 1 and rep['max_concurrent'] >= 2 and rep['overlap_s'] > 0
+    for j in alone_err:
+        assert mix_err[j] == alone_err[j], j
+        a, b = mix_state[j], alone_state[j]
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), j
+        for x, y in zip(a[2] + a[3], b[2] + b[3]):
+            assert np.array_equal(x, y), j
