@@ -266,6 +266,9 @@ PosArgs base_args(mset_ctx* c) {
     A.planes = plane_in_global(c->n) ? c->ws->planes.p : nullptr;
     A.sep = 0;
     A.p1d = c->p1d.p;
+    // the separable routine's phases share the transmission buffer: forward layouts, then adjoint layouts
+    A.tph = (const float*)c->ws->trans.p;
+    A.tph_b = A.tph + (size_t)c->L * plane(c);
     return A;
 }
 
@@ -309,7 +312,8 @@ int launch_grid(int want, int count, int S) {
 int ensure_fixed_buffers(mset_ctx* c) {
     Workspace& W = *c->ws;
     // the separable kernels keep the transmission in two layouts (forward and adjoint passes)
-    CHK(W.trans.ensure(plane(c) * c->L * (plane_in_global(c->n) ? 2 : 1), "transmission"));
+    // complex planes, or (separable routine) f32 phases in two layouts: L planes either way
+    CHK(W.trans.ensure(plane(c) * c->L, "transmission"));
     CHK(W.probeF.ensure(plane(c), "probe spectrum"));
     // the position kernels of the largest planes keep the plane in global memory: 2 planes per slot
     if (plane_in_global(c->n)) CHK(W.planes.ensure((size_t)batch_slots(c) * 2 * plane(c), "plane buffers"));
@@ -360,7 +364,7 @@ int ensure_batch_buffers(mset_ctx* c) {
 int prepare_fixed(mset_ctx* c, int sep = 0) {
     CHK(ensure_fixed_buffers(c));
     if (sep)
-        HIPCHK(launch_transmission_sep(c->V.p, c->ws->trans.p, sep == 2 ? c->ws->trans.p + plane(c) * c->L : nullptr,
+        HIPCHK(launch_transmission_sep(c->V.p, (float*)c->ws->trans.p, sep == 2 ? (float*)c->ws->trans.p + plane(c) * c->L : nullptr,
                                        c->n, c->nz, binning(c), c->p.sigma, c->p.potential_pixelsize, c->stream));
     else
         HIPCHK(launch_transmission(c->V.p, c->ws->trans.p, c->n, c->nz, binning(c), sset(c) ? 1 : 0, c->p.sigma,
@@ -434,7 +438,6 @@ int issue_grad(mset_ctx* c, int first, int count, bool apply, int want) {
     A.want = want;
     A.nslab = S;
     A.sep = sep ? 1 : 0;
-    A.trans_b = W.trans.p + plane(c) * c->L;
     // every used slab receives at least one position, and its first one writes it
     {
         Timed t(c, "position_batch");
@@ -800,7 +803,7 @@ int mset_grad_noalloc(mset_ctx* c, int first, int count, float* dV, float* dprob
     // buffers.  Four blocks with the kernels' plane strides, each starting on a 256-byte boundary.
     // the separable kernels (as mset_grad picks them) keep the transmission in two layouts
     const bool sep = sep_fwd(c) && c->p.grad_mode == GRAD_REF;
-    const size_t b_trans = up((sep ? 2 : 1) * L * n2 * sizeof(float2)), b_pf = up(n2 * sizeof(float2));
+    const size_t b_trans = up(L * n2 * sizeof(float2)), b_pf = up(n2 * sizeof(float2));
     const size_t s_g = L * n2 * sizeof(float), s_ps = n2 * sizeof(float2);
     const size_t s_save = (L + 2) * n2 * sizeof(float2), s_pl = plane_in_global(c->n) ? 2 * n2 * sizeof(float2) : 0;
     const size_t fixed = b_trans + b_pf + 4 * 256;
@@ -826,7 +829,7 @@ int mset_grad_noalloc(mset_ctx* c, int first, int count, float* dV, float* dprob
     float2* save = (float2*)q;   q += up((size_t)G * s_save);
     float2* planes_ = s_pl ? (float2*)q : nullptr;
     if (sep)
-        HIPCHK(launch_transmission_sep(c->V.p, trans, trans + L * n2, c->n, c->nz, binning(c), c->p.sigma,
+        HIPCHK(launch_transmission_sep(c->V.p, (float*)trans, (float*)trans + L * n2, c->n, c->nz, binning(c), c->p.sigma,
                                        c->p.potential_pixelsize, c->stream));
     else
         HIPCHK(launch_transmission(c->V.p, trans, c->n, c->nz, binning(c), 0, c->p.sigma, c->p.potential_pixelsize,
@@ -848,7 +851,8 @@ int mset_grad_noalloc(mset_ctx* c, int first, int count, float* dV, float* dprob
     A.want = WANT_PROBE | WANT_POS;
     A.nslab = S;
     A.sep = sep ? 1 : 0;
-    A.trans_b = trans + L * n2;
+    A.tph = (const float*)trans;
+    A.tph_b = (const float*)trans + L * n2;
     HIPCHK(launch_batch(c->n, A, G, c->stream));
     HIPCHK(launch_reduce_slabs(gslab, S, L * n2, dV, c->stream));
     if (sep) HIPCHK(launch_untranspose_odd(dV, c->n, c->L, c->stream));
@@ -1188,13 +1192,14 @@ int mset_recon_error_pass(mset_recon* r, double* errs) {
         CHK(c->mt_trans.ensure((size_t)c->L * plane(c), "tilt transmission"));
         CHK(c->mt_probeF.ensure(plane(c), "tilt probe spectrum"));
         if (sep)
-            HIPCHK(launch_transmission_sep(c->V.p, c->mt_trans.p, nullptr, c->n, c->nz, binning(c), c->p.sigma,
+            HIPCHK(launch_transmission_sep(c->V.p, (float*)c->mt_trans.p, nullptr, c->n, c->nz, binning(c), c->p.sigma,
                                            c->p.potential_pixelsize, r->stream));
         else
             HIPCHK(launch_transmission(c->V.p, c->mt_trans.p, c->n, c->nz, binning(c), 0, c->p.sigma,
                                        c->p.potential_pixelsize, r->stream));
         HIPCHK(launch_plane_fft(c->n, c->probe.p, c->mt_probeF.p, c->tw.p, 0, 1.f, 0.f, r->ws->planes.p, r->stream));
-        tab[j] = TiltArgs{c->mt_trans.p, c->mt_probeF.p, c->sqrtM.p, c->x.p, c->y.p, r->err_all.p + off[j]};
+        tab[j] = TiltArgs{c->mt_trans.p, (const float*)c->mt_trans.p, c->mt_probeF.p, c->sqrtM.p, c->x.p, c->y.p,
+                          r->err_all.p + off[j]};
         for (int k = 0; k < c->npos; ++k) items[off[j] + k] = (j << ITEM_TILT_SHIFT) | k;   // tilt-major
     }
     HIPCHK(hipMemcpyAsync(r->tilt_tab.p, tab.data(), sizeof(TiltArgs) * T, hipMemcpyHostToDevice, r->stream));

@@ -45,10 +45,12 @@ __global__ void k_trans(const float* __restrict__ V, float2* __restrict__ T, int
     }
 }
 
-// Transmission planes for the separable-propagator kernels: plane a goes to Tf in the layout of the pass that
-// applies it in the forward sweep (rows, [x][y], for even a; columns, [y][x], for odd a) and, if Tb is
-// given, to Tb in the layout of its adjoint pass (the other one).
-__global__ void k_trans_sep(const float* __restrict__ V, float2* __restrict__ Tf, float2* __restrict__ Tb, int n, int nz,
+// Transmission phases for the separable-propagator kernels, which form t = exp(i pi r) where they apply it
+// (half the bytes of a complex plane).  r is the f32 phase of k_trans, divided by pi and reduced to [-1, 1]
+// in double: t differs from k_trans's sincosf by about 1e-7.  Plane a goes to Tf in the
+// layout of the pass that applies it in the forward sweep (rows, [x][y], for even a; columns, [y][x], for
+// odd a) and, if Tb is given, to Tb in the layout of its adjoint pass (the other one).
+__global__ void k_trans_sep(const float* __restrict__ V, float* __restrict__ Tf, float* __restrict__ Tb, int n, int nz,
                             int bin, int L, float sigma, float dx) {
     const long long n2 = (long long)n * n, tot = n2 * L;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < tot; i += (long long)gridDim.x * blockDim.x) {
@@ -58,12 +60,11 @@ __global__ void k_trans_sep(const float* __restrict__ V, float2* __restrict__ Tf
         float s = 0.f;
         for (int z = z0; z < z1; ++z) s += V[(size_t)z * n2 + p];
         const float ph = sigma * (s * dx);                      // STEP02_FORWARD_4D.m:18-19 rounding order
-        float sn, cs;
-        sincosf(ph, &sn, &cs);
-        const float2 t = make_float2(cs, sn);
+        double r = (double)ph / M_PI;
+        r -= 2.0 * rint(0.5 * r);                               // [-1, 1]
         const long long rowmaj = (long long)y * n + x, colmaj = (long long)x * n + y;
-        Tf[(long long)a * n2 + ((a & 1) ? rowmaj : colmaj)] = t;
-        if (Tb) Tb[(long long)a * n2 + ((a & 1) ? colmaj : rowmaj)] = t;
+        Tf[(long long)a * n2 + ((a & 1) ? rowmaj : colmaj)] = (float)r;
+        if (Tb) Tb[(long long)a * n2 + ((a & 1) ? colmaj : rowmaj)] = (float)r;
     }
 }
 
@@ -156,7 +157,7 @@ hipError_t launch_transmission(const float* V, float2* T, int n, int nz, int bin
     return hipGetLastError();
 }
 
-hipError_t launch_transmission_sep(const float* V, float2* Tf, float2* Tb, int n, int nz, int bin, float sigma, float dx,
+hipError_t launch_transmission_sep(const float* V, float* Tf, float* Tb, int n, int nz, int bin, float sigma, float dx,
                                    hipStream_t st) {
     const int L = (nz + bin - 1) / bin;
     hipLaunchKernelGGL(k_trans_sep, grid_for((size_t)n * n * L), dim3(256), 0, st, V, Tf, Tb, n, nz, bin, L, sigma, dx);

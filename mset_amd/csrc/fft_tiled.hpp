@@ -251,11 +251,17 @@ template <int N, int NT_> struct TiledEngine {
         } else if constexpr (IsTileStep<M>::value) {
             m.template run<COL>(*this, v, l0, line, j);
         } else if constexpr (IsStaged<M>::value) {
-            decltype(m.load(Pt{})) op[R];
+            // operands in groups of G elements: each group's loads are issued back to back before its applies
+            constexpr int G = M::G < R ? M::G : R;
+            static_assert(R % G == 0, "staging group must divide the thread's elements");
 #pragma unroll
-            for (int r = 0; r < R; ++r) op[r] = m.load(pt<COL>(l0, line, j + R * r));
+            for (int r0 = 0; r0 < R; r0 += G) {
+                decltype(m.load(Pt{})) op[G];
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[r] = m.apply(pt<COL>(l0, line, j + R * r), v[r], op[r]);
+                for (int r = 0; r < G; ++r) op[r] = m.load(pt<COL>(l0, line, j + R * (r0 + r)));
+#pragma unroll
+                for (int r = 0; r < G; ++r) v[r0 + r] = m.apply(pt<COL>(l0, line, j + R * (r0 + r)), v[r0 + r], op[r]);
+            }
         } else {
 #pragma unroll
             for (int r = 0; r < R; ++r) v[r] = m(pt<COL>(l0, line, j + R * r), v[r]);

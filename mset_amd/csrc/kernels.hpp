@@ -10,7 +10,8 @@ enum { WANT_PROBE = 1, WANT_POS = 2 };
 // Per-tilt inputs and outputs of a launch that runs the positions of several tilts (a table in device
 // memory; position k of tilt j is the work item (j << 24) | k).
 struct TiltArgs {
-    const float2* trans;         // transmission planes of the tilt's rotated volume
+    const float2* trans;         // transmission planes of the tilt's rotated volume (general routine)
+    const float* tph;            // or their phases in the forward layouts (separable routine)
     const float2* probeF;        // fftn(the tilt's probe)
     const float* sqrtM;          // the tilt's patterns (sqrt, ifftshifted, transposed)
     float* xlist;                // the tilt's scan positions
@@ -49,10 +50,11 @@ struct PosArgs {
     float* gslab;                // object-gradient slabs, L planes each (K_PLAIN: slab 0, this position)
     float2* pslab;               // K_BATCH: probe-gradient slabs, one plane each (Fourier domain)
     int nslab;                   // K_BATCH: position p of the launch adds to slab p % nslab
-    int sep;                     // N = 256: separable-propagator routine (p1d, trans in the forward layouts,
-                                 // trans_b in the adjoint layouts; odd gradient planes transposed)
+    int sep;                     // N = 256: separable-propagator routine (p1d, tph in the forward layouts,
+                                 // tph_b in the adjoint layouts; odd gradient planes transposed)
     const float2* p1d;           // sep: the propagator's 1D factors px[N], py[N] (p(kx, ky) = px(kx) py(ky))
-    const float2* trans_b;       // sep, K_BATCH: transmission planes in the adjoint passes' layouts
+    const float* tph;            // sep: transmission phases / pi, reduced to [-1, 1] (t = exp(i pi tph)), forward layouts
+    const float* tph_b;          // sep, K_BATCH: the same phases in the adjoint passes' layouts
     float2* planes;              // N = 256: per-slot plane buffers (2 N^2), the tiled engine's state
     float* dpos;                 // K_BATCH: per-position update terms (2 per position)
 };
@@ -77,9 +79,9 @@ hipError_t launch_ingest_patterns(const float* M, float* sqrtM, int n, long long
 hipError_t launch_transmission(const float* V, float2* trans, int n, int nz, int bin, int sset, float sigma, float dx,
                                hipStream_t st);
 hipError_t launch_fill(void* p, size_t bytes, hipStream_t st);
-// separable-propagator kernels: transmission planes in the forward (Tf) and adjoint (Tb, nullable) layouts,
+// separable-propagator kernels: transmission phases in the forward (Tf) and adjoint (Tb, nullable) layouts,
 // and the in-place fix of the odd gradient planes they leave transposed
-hipError_t launch_transmission_sep(const float* V, float2* Tf, float2* Tb, int n, int nz, int bin, float sigma, float dx,
+hipError_t launch_transmission_sep(const float* V, float* Tf, float* Tb, int n, int nz, int bin, float sigma, float dx,
                                    hipStream_t st);
 hipError_t launch_untranspose_odd(float* G, int n, int L, hipStream_t st);
 hipError_t launch_reduce_slabs(const float* slab, int nslab, size_t n, float* out, hipStream_t st);

@@ -72,6 +72,12 @@ struct AdjOp {
     float2 t, w;
     float g;
 };
+// the same with the transmission as its phase (separable routine)
+struct AdjPh {
+    float ph;
+    float2 w;
+    float g;
+};
 
 
 // slot: the workgroup's scratch (waves, plane buffers); slab: the gradient slab the position adds to;
@@ -414,12 +420,14 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
 // puts two half-steps along the same axis around every transmission multiply, so each chain pass runs
 // [F, x p, F^-1, (store psi_a, x t_a), F, x p, F^-1] along one axis: a plane costs one pass over the
 // global plane buffers instead of two.  The adjoint does the same with conj(p) and the gradient step.
-// The 1D factors live in LDS, so the propagator costs no global loads.
+// The 1D factors live in LDS, so the propagator costs no global loads.  The transmission is read as its
+// phase r (in units of pi, reduced to [-1, 1]) and formed where it is applied, t = exp(i pi r): half the
+// bytes of a complex plane.
 //
 // Orientation of pass X_k: columns (along y) for even k, rows (along x) for odd k.  The forward step of
 // plane a is in X_(a+1), the adjoint step in X_(2L+2-a): the reference pairing conj(psi_(a+1)) reads each
 // stored wave in the orientation that wrote it, while the transmission of plane a is read in opposite
-// orientations, so it is kept in both layouts (trans: forward, trans_b: adjoint).  Gradient slab planes
+// orientations, so it is kept in both layouts (tph: forward, tph_b: adjoint).  Gradient slab planes
 // are in their adjoint pass's layout (odd planes transposed; the host puts them back after the sum).
 // Arrays kept in the other layout than the pass that reads them (fftn(probe) and sqrt(M), [x][y], in a
 // column pass; psi0, [y][x], written in a row pass) go through the LDS tile (TileLoad / TileStore).
@@ -457,6 +465,13 @@ __device__ __forceinline__ void process_sep(const PosArgs& A, int k, int slot, i
     }
     __syncthreads();
     auto phase = [&](int y, int x) { return cmul(phy[y], phx[x]); };
+    // exp(i pi r), r in [-1, 1]: v_cos_f32 / v_sin_f32 take revolutions (r / 2 in [-0.5, 0.5]); their error
+    // there is at most 1.3e-7 absolute (4M points against double), two float ulps at magnitude 1.  sincospif
+    // (5e-8) costs more VALU time than the halved transmission bytes save.
+    auto expi = [](float r) {
+        const float h = 0.5f * r;
+        return make_float2(__builtin_amdgcn_cosf(h), __builtin_amdgcn_sinf(h));
+    };
 
     // propagation factors along the pass axis, with the 1/N of that axis' inverse transform
     auto pfac = [&](auto colt) {
@@ -476,20 +491,20 @@ __device__ __forceinline__ void process_sep(const PosArgs& A, int k, int slot, i
     }));
     // ---- X1 (rows): inverse along x -> psi0 (stored [y][x] through the LDS tile), x t_0, half-step along x
     {
-        const BufC t0(A.trans, N2);
+        const BufF t0(A.tph, N2);
         auto scale = [&](const Pt&, float2 v) { return cscale(v, inv_n2); };
-        auto times_t = staged<16>([&](const Pt& p) { return t0[p.e]; }, [&](const Pt&, float2 v, float2 o) { return cmul(v, o); });
+        auto times_t = staged<16>([&](const Pt& p) { return t0[p.e]; }, [&](const Pt&, float2 v, float o) { return cmul(v, expi(o)); });
         E.template chain<false, SWAP, INV, FWD, INV>(NoOp(), seq(scale, TileStore{GRAD ? save : nullptr}, times_t),
                                                      pfac(RT{}));
     }
     // ---- X_(a+1), a = 1 .. L-1: finish plane a-1's step, store psi_a, x t_a, start plane a's step
     auto fwd_pass = [&](auto colt, int a) {
         constexpr bool COL = decltype(colt)::value;
-        const BufC ta(A.trans + (size_t)a * N2, N2);
+        const BufF ta(A.tph + (size_t)a * N2, N2);
         const BufC sv(save + (size_t)a * N2, N2);
-        auto mid = staged<16>([&](const Pt& p) { return ta[p.e]; }, [&](const Pt& p, float2 v, float2 o) {
+        auto mid = staged<16>([&](const Pt& p) { return ta[p.e]; }, [&](const Pt& p, float2 v, float o) {
             if (GRAD) sv.put(p.e, v);
-            return cmul(v, o);
+            return cmul(v, expi(o));
         });
         E.template chain<COL, SWAP, FWD, INV, FWD, INV>(NoOp(), pfac(colt), mid, pfac(colt));
     };
@@ -545,13 +560,13 @@ __device__ __forceinline__ void process_sep(const PosArgs& A, int k, int slot, i
     double dr = 0.0, dc = 0.0;
     auto adj_pass = [&](auto colt, int a) {
         constexpr bool COL = decltype(colt)::value;
-        const BufC tb(A.trans_b + (size_t)a * N2, N2);
+        const BufF tb(A.tph_b + (size_t)a * N2, N2);
         const BufC ws(save + (size_t)(a + 1) * N2, N2);              // reference pairing: the wave after plane a
         const BufF g(A.gslab + ((size_t)slab * L + a) * N2, N2);
         const BufF gin(A.gslab + ((size_t)slab * L + a) * N2, first ? 0u : N2);   // empty view: reads 0
-        auto mid = staged<16>([&](const Pt& p) { return AdjOp{tb[p.e], ws[p.e], gin[p.e]}; },
-                              [&](const Pt& p, float2 u, const AdjOp& o) {
-                                  const float2 tc = cconj(o.t);
+        auto mid = staged<8>([&](const Pt& p) { return AdjPh{tb[p.e], ws[p.e], gin[p.e]}; },
+                              [&](const Pt& p, float2 u, const AdjPh& o) {
+                                  const float2 tc = cconj(expi(o.ph));
                                   const float2 q = cmul(tc, cmulc(u, o.w));
                                   g.put(p.e, o.g - q.y);                // Re(1i * conj(t) * conj(psi) * b)
                                   return cmul(u, tc);
@@ -677,6 +692,7 @@ __global__ __launch_bounds__(Cfg<N>::NT, Cfg<N>::MINB) void k_fwd(PosArgs A) {
             const TiltArgs& T = A.tilts[item >> ITEM_TILT_SHIFT];
             PosArgs B = A;
             B.trans = const_cast<float2*>(T.trans);
+            B.tph = T.tph;
             B.probeF = T.probeF;
             B.sqrtM = T.sqrtM;
             B.xlist = T.xlist;
