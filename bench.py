@@ -101,8 +101,8 @@ def main():
     ap.add_argument('--plain-tilts', type=int, default=1, help='tilts of the plain path to time (0 = skip); the '
                     'smallest tilts are taken')
     ap.add_argument('--cpu-positions', type=int, default=None,
-                    help='positions of the all-cores CPU sample (0 = skip both CPU samples; default 48 for C3, '
-                         '256 for C2); the one-thread sample takes a third of them')
+                    help='positions of the all-cores CPU sample (0 = skip both CPU samples; default 120 for C3, '
+                         '1024 for C2, about 10 s each); the one-thread sample takes a third of them')
     ap.add_argument('--cpu-threads', type=int, default=None, help='threads of the all-cores sample (default: the '
                     'cores this process may use)')
     args = ap.parse_args()
@@ -120,7 +120,7 @@ def main():
     elif args.no_autotune:
         batches = (512 if big else 1024,)
     if args.cpu_positions is None:
-        args.cpu_positions = 48 if big else 256
+        args.cpu_positions = 120 if big else 1024
     if args.cpu_threads is None:
         args.cpu_threads = host_threads()
 

@@ -131,12 +131,18 @@ int mset_waves(mset_ctx* ctx, int k, float* waves_out);
  * resident position, the reference's save_wave3D); RECOMPUTE keeps 5 and rebuilds psi_a in the adjoint by
  * inverting the forward sweep, psi_a = conj(t_a) .* ifftn(fftn(psi_{a+1}) .* conj(prop)), for 50 % more
  * transforms.  It needs a unit-modulus propagator (the Fresnel propagator of Func_generate_propagator.m
- * is one; EINVAL otherwise).  AUTO (the default) stores unless the store scratch (waves and gradient
- * slabs) would exceed a quarter of the device memory.  No counterpart in the reference, which stores.
- * mset_exit_waves reports the variant and the per-position scratch bytes of the last batched call. */
+ * is one; EINVAL otherwise).  CHECKPOINT keeps every K-th wave (K = ceil(sqrt(L + 1))) and the last
+ * segment of K, L / K + K + 2 planes, and rebuilds each earlier segment from its checkpoint with the
+ * forward sweep's own steps: any propagator, the stored variant's waves, about one forward sweep more.
+ * AUTO (the default) stores unless the store scratch (waves and gradient slabs) would exceed a quarter of
+ * the device memory; then it checkpoints if that scratch fits, else recomputes (unit-modulus propagator),
+ * else checkpoints.  RECOMPUTE with fewer than 4 planes, and every variant under the single-slice operator, store.
+ * No counterpart in the reference, which stores.  mset_exit_waves reports the variant and the per-position
+ * scratch bytes of the last batched call. */
 #define MSET_EXIT_WAVES_AUTO 0
 #define MSET_EXIT_WAVES_STORE 1
 #define MSET_EXIT_WAVES_RECOMPUTE 2
+#define MSET_EXIT_WAVES_CHECKPOINT 3
 int mset_set_exit_waves(mset_ctx* ctx, int mode);
 int mset_exit_waves(mset_ctx* ctx, int* resolved, size_t* bytes_per_slot);
 /* Cap on the persistent grid of the batched kernels (resident positions, each with its own scratch);

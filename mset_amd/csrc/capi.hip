@@ -88,7 +88,7 @@ struct Workspace {
     int exit_waves = MSET_EXIT_WAVES_AUTO;   // requested exit-wave variant of the batched kernels
     int slot_limit = 0;                      // cap on the persistent grid of the batched kernels (0: none)
     int engine = MSET_ENGINE_AUTO;           // plane routine of the 256 x 256 kernels
-    bool recompute = false;                  // the variant of the last batched call
+    int waves = WAVES_STORE;                 // the variant of the last batched call (WAVES_*)
     size_t slot_bytes = 0;                   // its per-slot scratch (waves + gradient slab)
     RotWork rot;
     TvWork tv;
@@ -259,7 +259,8 @@ PosArgs base_args(mset_ctx* c) {
     A.ylist = c->y.p;
     A.err_pos = c->err.p;
     A.save = c->ws->save.p;
-    A.recompute = c->ws->recompute ? 1 : 0;
+    A.waves = c->ws->waves;
+    A.ckseg = checkpoint_segment(planes(c));
     A.gslab = c->ws->gslab.p;
     A.pslab = c->ws->pslab.p;
     A.dpos = c->dpos.p;
@@ -277,7 +278,7 @@ PosArgs base_args(mset_ctx* c) {
 bool sep_fwd(const mset_ctx* c) {
     return plane_in_global(c->n) && c->prop_sep && !sset(c) && c->ws->engine != MSET_ENGINE_GENERAL;
 }
-bool sep_batch(const mset_ctx* c) { return sep_fwd(c) && c->p.grad_mode == GRAD_REF && !c->ws->recompute; }
+bool sep_batch(const mset_ctx* c) { return sep_fwd(c) && c->p.grad_mode == GRAD_REF && c->ws->waves == WAVES_STORE; }
 
 // workgroups of the batched kernels the device keeps resident at once (1 per CU at N = 128)
 int device_slots(mset_ctx* c) {
@@ -320,21 +321,38 @@ int ensure_fixed_buffers(mset_ctx* c) {
     return MSET_OK;
 }
 
-// exit-wave variant of a batched call (see mset_set_exit_waves): recompute only where it saves memory
-// (more than RECOMPUTE_PLANES planes stored) and the propagator is unit-modulus
-int resolve_exit_waves(mset_ctx* c, int slots, bool* recompute) {
+// exit-wave variant of a batched call (see mset_set_exit_waves).  AUTO stores unless the store scratch would
+// exceed a quarter of the device; then it checkpoints if that scratch fits (faster than recomputing and the
+// stored variant's waves), else recomputes (unit-modulus propagator), else checkpoints.  RECOMPUTE applies
+// from 4 planes on; the single-slice operator always stores.
+int wave_planes(const mset_ctx* c, int waves) {
+    const int L = planes(c);
+    return waves == WAVES_RECOMPUTE ? RECOMPUTE_PLANES
+           : waves == WAVES_CHECKPOINT ? checkpoint_planes(L, checkpoint_segment(L)) : L + 2;
+}
+
+int resolve_exit_waves(mset_ctx* c, int slots, int* waves) {
     Workspace& W = *c->ws;
     const int L = planes(c);
-    const bool useful = !sset(c) && L + 2 > RECOMPUTE_PLANES;
-    *recompute = false;
+    *waves = WAVES_STORE;
+    if (sset(c)) return MSET_OK;
     if (W.exit_waves == MSET_EXIT_WAVES_RECOMPUTE) {
         if (!c->prop_unitary) return fail(MSET_EINVAL, "recomputed exit waves need a unit-modulus propagator");
-        *recompute = useful;
-    } else if (W.exit_waves == MSET_EXIT_WAVES_AUTO && useful && c->prop_unitary) {
+        if (L + 2 > RECOMPUTE_PLANES) *waves = WAVES_RECOMPUTE;
+    } else if (W.exit_waves == MSET_EXIT_WAVES_CHECKPOINT) {
+        *waves = WAVES_CHECKPOINT;
+    } else if (W.exit_waves == MSET_EXIT_WAVES_AUTO) {
         size_t fr = 0, total = 0;
         HIPCHK(hipMemGetInfo(&fr, &total));
-        const double store = (double)slots * (double)plane(c) * ((L + 2) * sizeof(float2) + L * sizeof(float));
-        *recompute = store > 0.25 * (double)total;
+        auto scratch = [&](int wv) {
+            return (double)slots * (double)plane(c) * (wave_planes(c, wv) * sizeof(float2) + L * sizeof(float));
+        };
+        const double quarter = 0.25 * (double)total;
+        if (scratch(WAVES_STORE) > quarter) {
+            int alt = WAVES_CHECKPOINT;
+            if (scratch(WAVES_CHECKPOINT) > quarter && c->prop_unitary && L + 2 > RECOMPUTE_PLANES) alt = WAVES_RECOMPUTE;
+            if (L + 2 > wave_planes(c, alt)) *waves = alt;
+        }
     }
     return MSET_OK;
 }
@@ -343,11 +361,11 @@ int ensure_batch_buffers(mset_ctx* c) {
     Workspace& W = *c->ws;
     const int s = batch_slots(c);
     CHK(ensure_fixed_buffers(c));
-    bool rc = false;
-    CHK(resolve_exit_waves(c, s, &rc));
-    const int save_planes = rc ? RECOMPUTE_PLANES : c->L + 2;
+    int wv = WAVES_STORE;
+    CHK(resolve_exit_waves(c, s, &wv));
+    const int save_planes = wave_planes(c, wv);
     CHK(W.save.ensure((size_t)s * save_planes * plane(c), "save waves"));
-    W.recompute = rc;
+    W.waves = wv;
     W.slot_bytes = plane(c) * (save_planes * sizeof(float2) + (size_t)planes(c) * sizeof(float));
     const int S = grad_slabs(c);
     CHK(W.gslab.ensure((size_t)S * c->L * plane(c), "gradient slabs"));
@@ -839,7 +857,7 @@ int mset_grad_noalloc(mset_ctx* c, int first, int count, float* dV, float* dprob
     A.trans = trans;
     A.probeF = probeF;
     A.save = save;
-    A.recompute = 0;
+    A.waves = WAVES_STORE;
     A.gslab = gslab;
     A.pslab = pslab;
     A.planes = planes_;
@@ -873,7 +891,7 @@ int mset_waves(mset_ctx* c, int k, float* waves_out) {
     A.first = k;
     A.count = 1;
     A.want = 0;
-    A.recompute = 0;
+    A.waves = WAVES_STORE;
     A.nslab = 1;
     HIPCHK(launch_batch(c->n, A, 1, c->stream));
     HIPCHK(hipMemcpyAsync(waves_out, c->ws->save.p, sizeof(float2) * (size_t)(c->L + 1) * plane(c),
@@ -894,7 +912,8 @@ int mset_run_batched(mset_ctx* c, int method, int batch, double* err_out) {
 
 int mset_set_exit_waves(mset_ctx* c, int mode) {
     CHK(check_ctx(c));
-    if (mode != MSET_EXIT_WAVES_AUTO && mode != MSET_EXIT_WAVES_STORE && mode != MSET_EXIT_WAVES_RECOMPUTE)
+    if (mode != MSET_EXIT_WAVES_AUTO && mode != MSET_EXIT_WAVES_STORE && mode != MSET_EXIT_WAVES_RECOMPUTE &&
+        mode != MSET_EXIT_WAVES_CHECKPOINT)
         return fail(MSET_EINVAL, "bad exit-wave mode");
     if (mode != c->ws->exit_waves) {
         // the next batched call sizes the wave scratch for the new variant
@@ -921,7 +940,10 @@ int mset_set_engine(mset_ctx* c, int engine) {
 
 int mset_exit_waves(mset_ctx* c, int* resolved, size_t* bytes_per_slot) {
     CHK(check_ctx(c));
-    if (resolved) *resolved = c->ws->recompute ? MSET_EXIT_WAVES_RECOMPUTE : MSET_EXIT_WAVES_STORE;
+    if (resolved)
+        *resolved = c->ws->waves == WAVES_RECOMPUTE    ? MSET_EXIT_WAVES_RECOMPUTE
+                    : c->ws->waves == WAVES_CHECKPOINT ? MSET_EXIT_WAVES_CHECKPOINT
+                                                       : MSET_EXIT_WAVES_STORE;
     if (bytes_per_slot) *bytes_per_slot = c->ws->slot_bytes;
     return MSET_OK;
 }
@@ -1261,7 +1283,8 @@ int mset_recon_get(mset_recon* r, float* rec, float* probes, float* xlists, floa
 int mset_recon_set_exit_waves(mset_recon* r, int mode) {
     if (!r) return fail(MSET_EINVAL, "null reconstruction");
     HIPCHK(hipSetDevice(r->device));
-    if (mode != MSET_EXIT_WAVES_AUTO && mode != MSET_EXIT_WAVES_STORE && mode != MSET_EXIT_WAVES_RECOMPUTE)
+    if (mode != MSET_EXIT_WAVES_AUTO && mode != MSET_EXIT_WAVES_STORE && mode != MSET_EXIT_WAVES_RECOMPUTE &&
+        mode != MSET_EXIT_WAVES_CHECKPOINT)
         return fail(MSET_EINVAL, "bad exit-wave mode");
     if (mode != r->ws->exit_waves) {
         HIPCHK(hipStreamSynchronize(r->stream));

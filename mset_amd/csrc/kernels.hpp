@@ -29,7 +29,8 @@ struct PosArgs {
     float s1, s2, s3;            // step sizes (object, probe, position)
     int grad_mode;               // GRAD_REF or GRAD_EXACT
     int sset;                    // single-slice operator: L = 1, bin = nz, no propagation, phase (sigma sum V) dx
-    int recompute;               // K_BATCH: rebuild the waves in the adjoint instead of storing every plane
+    int waves;                   // K_BATCH: WAVES_STORE, WAVES_RECOMPUTE or WAVES_CHECKPOINT (exit-wave variant)
+    int ckseg;                   // WAVES_CHECKPOINT: segment length K (checkpoint_segment)
     int want;                    // WANT_PROBE | WANT_POS
     const float2* tw;            // W_N^k, k < N
     const float2* prop;          // propagator, unshifted
@@ -46,7 +47,7 @@ struct PosArgs {
     int obase;                   // err_pos and dpos are indexed k - obase (0: by position index)
     double* err_pos;             // per position: sum over pixels of (sqrt M - sqrt I)^2
     float* inten;                // K_FWD, optional: |Psi|^2 per position, DC-centred (MATLAB layout)
-    float2* save;                // per-slot scratch: (L + 2) planes, or RECOMPUTE_PLANES when recomputing
+    float2* save;                // per-slot scratch: L + 2 planes, RECOMPUTE_PLANES or checkpoint_planes(L)
     float* gslab;                // object-gradient slabs, L planes each (K_PLAIN: slab 0, this position)
     float2* pslab;               // K_BATCH: probe-gradient slabs, one plane each (Fourier domain)
     int nslab;                   // K_BATCH: position p of the launch adds to slab p % nslab
@@ -59,8 +60,21 @@ struct PosArgs {
     float* dpos;                 // K_BATCH: per-position update terms (2 per position)
 };
 
+// Exit-wave variants of the batched kernel.  STORE keeps every wave psi_0 .. psi_L; RECOMPUTE keeps five
+// planes and rebuilds psi_a in the adjoint by inverting the forward step (unit-modulus propagator);
+// CHECKPOINT keeps every K-th wave and the last segment, and rebuilds each earlier segment of K waves from
+// its checkpoint by running the forward steps again (any propagator).
+enum { WAVES_STORE = 0, WAVES_RECOMPUTE = 1, WAVES_CHECKPOINT = 2 };
 // per-slot scratch planes of the recompute variant: psi0, Fourier psi0, two wave planes, parked adjoint
 constexpr int RECOMPUTE_PLANES = 5;
+// checkpoint variant: segment length K = ceil(sqrt(L + 1)) and its per-slot planes: psi0, Fourier psi0, the
+// checkpoints psi_K, psi_2K, .. psi_(L / K) K, the K - 1 other waves of one segment, the parked adjoint wave
+__host__ __device__ inline int checkpoint_segment(int L) {
+    int k = 2;
+    while (k * k < L + 1) ++k;
+    return k;
+}
+__host__ __device__ inline int checkpoint_planes(int L, int K) { return L / K + K + 2; }
 
 bool plane_size_supported(int n);
 int position_threads(int n);

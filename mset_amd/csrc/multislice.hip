@@ -83,7 +83,7 @@ struct AdjPh {
 // slot: the workgroup's scratch (waves, plane buffers); slab: the gradient slab the position adds to;
 // first: the slab's first position of the launch, which writes the slab instead of adding to it (so the
 // slabs need no clearing launch)
-template <int N, int NT, int KIND, bool RC = false, class Eng>
+template <int N, int NT, int KIND, int WV = WAVES_STORE, class Eng>
 __device__ __forceinline__ void process_position(const PosArgs& A, int k, int slot, int slab, bool first, Eng& E,
                                                  float2* phx, float2* phy, double* red) {
     constexpr int N2 = N * N;
@@ -93,9 +93,19 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
     // Per-slot scratch.  Store: planes 0..L hold the waves psi_a [y][x], plane L+1 the Fourier psi0 [x][y].
     // Recompute (batched kernel only): plane 0 psi0, plane 1 Fourier psi0, planes 2 and 3 the wave on
     // either side of the current adjoint plane, plane 4 the parked adjoint wave.
-    constexpr bool recompute = RC && KIND == K_BATCH;
-    float2* save = A.save + (size_t)slot * (recompute ? RECOMPUTE_PLANES : L + 2) * N2;
-    float2* F0 = save + (size_t)(recompute ? 1 : L + 1) * N2;
+    // Checkpoint (batched kernel only): plane 0 psi0, plane 1 Fourier psi0, planes 1 + s the checkpoints
+    // psi_(s K), s = 1 .. L / K, planes 1 + L / K + m the waves psi_(s K + m), m = 1 .. K - 1, of the segment
+    // s the adjoint is in, and the last plane the parked adjoint wave.
+    constexpr bool recompute = WV == WAVES_RECOMPUTE && KIND == K_BATCH;
+    constexpr bool ckpt = WV == WAVES_CHECKPOINT && KIND == K_BATCH;
+    const int K = ckpt ? A.ckseg : 1, SL = L / K;
+    float2* save = A.save + (size_t)slot * (recompute ? RECOMPUTE_PLANES : ckpt ? checkpoint_planes(L, K) : L + 2) * N2;
+    float2* F0 = save + (size_t)(recompute || ckpt ? 1 : L + 1) * N2;
+    // checkpoint variant: where wave psi_i lives (a checkpoint, psi0, or the segment buffer)
+    auto ck_wave = [&](int i) -> float2* {
+        const int m = i % K;
+        return save + (size_t)(m != 0 ? 1 + SL + m : i == 0 ? 0 : 1 + i / K) * N2;
+    };
     E.bind(A.planes ? A.planes + (size_t)slot * 2 * N2 : nullptr);
     const bool want_pos = (A.want & WANT_POS) != 0;
     const bool want_probe = (A.want & WANT_PROBE) != 0;
@@ -180,9 +190,10 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
         if constexpr (FUSE) E.row_step(byp);
         else E.fwd(NoOp(), byp);
         const bool next = a + 1 < L;
-        // recompute keeps only the exit wave psi_L (plane 2), the start of the adjoint's rebuild
-        const bool keep = KIND != K_FWD && (!recompute || !next);
-        const BufC sv(save + (size_t)(recompute ? 2 : a + 1) * N2, N2);
+        // recompute keeps only the exit wave psi_L (plane 2), the start of the adjoint's rebuild; checkpoint
+        // keeps psi_(a+1) if it is a checkpoint or in the last segment, which the adjoint meets first
+        const bool keep = KIND != K_FWD && (recompute ? !next : ckpt ? ((a + 1) % K == 0 || (a + 1) / K == SL) : true);
+        const BufC sv(recompute ? save + 2 * (size_t)N2 : ckpt ? ck_wave(a + 1) : save + (size_t)(a + 1) * N2, N2);
         // the last plane has no successor: an empty view reads zeros and the product is skipped
         const BufC tn(A.trans + (size_t)(next ? a + 1 : 0) * N2, next ? N2 : 0u);
         auto post = staged<16, 32>([&](const Pt& p) { return tn[p.e]; }, [&](const Pt& p, float2 v, float2 o) {
@@ -234,16 +245,67 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
     // Recompute: the forward sweep is inverted plane by plane, psi_a = conj(t_a) .* ifftn(fftn(psi_{a+1})
     // .* conj(p)), exact because the propagator and the transmission have unit modulus (the host only
     // selects this variant for a unit-modulus propagator).  cur holds psi_{a+1} at the top of plane a.
+    // Checkpoint: the adjoint of plane a pairs psi_i (i = a + 1, or a in the exact mode), i descending; on
+    // entering segment s = i / K below the one held, the segment's waves psi_(s K + 1) .. psi_(s K + K - 1)
+    // are rebuilt from its checkpoint by the forward steps of planes s K .. s K + K - 2, the same arithmetic
+    // as the forward sweep's.  cseg is the segment held (the last one, stored by the forward sweep).
     float2* cur = save + 2 * (size_t)N2;
     float2* prev = save + 3 * (size_t)N2;
-    const BufC park(save + 4 * (size_t)N2, N2);
+    const BufC park(save + (size_t)(ckpt ? checkpoint_planes(L, K) - 1 : 4) * N2, N2);
+    int cseg = SL;
+    // checkpoint: the forward steps of segment cseg from its first wave, each wave stored
+    auto rebuild_segment = [&]() {
+        const int i0 = cseg * K;
+        const BufC cb(ck_wave(i0), N2), tb0(A.trans + (size_t)i0 * N2, N2);
+        auto enter = staged<16>([&](const Pt& p) { const float2 w = cb[p.e], tt = tb0[p.e]; return make_float4(w.x, w.y, tt.x, tt.y); },
+                                [&](const Pt&, float2, float4 o) { return cmul(make_float2(o.x, o.y), make_float2(o.z, o.w)); });
+        auto byp = staged<16, 32>([&](const Pt& p) { return pTb[p.e]; }, [&](const Pt&, float2 v, float2 o) { return cmul(v, o); });
+        if constexpr (FUSE) E.template col_first<true>(enter);
+        else E.template fwd<true>(enter, byp);
+        for (int a2 = i0; a2 < i0 + K - 1; ++a2) {
+            const bool more = a2 + 2 < i0 + K;                // another plane of the segment follows
+            const BufC sb(ck_wave(a2 + 1), N2);
+            const BufC tn(A.trans + (size_t)(more ? a2 + 1 : 0) * N2, more ? N2 : 0u);
+            auto post = staged<16, 32>([&](const Pt& p) { return tn[p.e]; }, [&](const Pt& p, float2 v, float2 o) {
+                const float2 w = cscale(v, inv_n2);
+                sb.put(p.e, w);
+                return more ? cmul(w, o) : w;
+            });
+            if constexpr (FUSE) {
+                E.row_step(byp);
+                if (more) E.col_step(post);
+                else E.col_last(post);
+            } else {
+                E.inv(NoOp(), post);
+                if (more) E.fwd(NoOp(), byp);
+            }
+        }
+    };
     for (int a = L - 1; a >= 0; --a) {
         const BufC ta(A.trans + (size_t)a * N2, N2);
         // the single-slice operator pairs the incident wave (STEP03_BACK_SSET.m:13-14)
         const float2* wsp = save + (size_t)((exact || A.sset) ? a : a + 1) * N2;
-        const bool rebuild = recompute && a > 0;
+        bool rebuild = recompute && a > 0;
         if (recompute) wsp = exact ? (a > 0 ? prev : save) : cur;
-        if constexpr (KIND == K_BATCH && FUSE) {
+        if constexpr (ckpt) {
+            const int i = exact ? a : a + 1;
+            rebuild = i / K != cseg;
+            if (rebuild) cseg = i / K;
+            wsp = ck_wave(i);
+        }
+        if constexpr (ckpt) {
+            if (rebuild) {
+                if constexpr (FUSE) {
+                    // the adjoint wave (half-inverse state) stays in its buffer; the rebuild runs on the park plane
+                    float2* held = E.swap_rs(save + (size_t)(checkpoint_planes(L, K) - 1) * N2);
+                    rebuild_segment();
+                    E.swap_rs(held);
+                } else {
+                    E.each_R([&](const Pt& p, float2& v) { park.put(p.e, v); });
+                    rebuild_segment();
+                }
+            }
+        } else if constexpr (KIND == K_BATCH && FUSE) {
             if (rebuild) {
                 // the adjoint wave (half-inverse state) stays in its buffer while the engine rebuilds psi_a
                 // into prev on the park plane: three fused passes, no copies
@@ -309,7 +371,7 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
                                                 return r;
                                             });
             if constexpr (FUSE) E.col_step(post);
-            else if (rebuild)
+            else if (rebuild)   // the parked adjoint wave comes back as the inverse transform's input
                 E.template inv<true>(staged<16>([&](const Pt& p) { return park[p.e]; }, [&](const Pt&, float2, float2 o) { return o; }),
                                      post);
             else E.inv(NoOp(), post);
@@ -323,7 +385,7 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
             if constexpr (FUSE) E.row_last(NoOp());
             else E.fwd();
         }
-        if (rebuild) {
+        if (recompute && rebuild) {
             float2* t2 = cur;
             cur = prev;
             prev = t2;
@@ -643,12 +705,12 @@ __global__ __launch_bounds__(Cfg<N>::NT, Cfg<N>::MINB) void k_plain(PosArgs A, i
     process_position<N, NT, K_PLAIN>(A, k, 0, 0, true, E, phx, phy, red);
 }
 
-// RC: the recompute variant (A.recompute), a separate instantiation so that the store variant carries
-// none of its code.  Position p of the launch adds to slab p % nslab.  The host launches either at most
+// WV: the exit-wave variant (A.waves), a separate instantiation each so that the store variant carries
+// none of the others' code.  Position p of the launch adds to slab p % nslab.  The host launches either at most
 // nslab positions or a grid that divides nslab, so every slab's positions run on one workgroup in
 // increasing p: each slab sums the same positions in the same order whatever the grid size.
 // SEP: the separable-propagator routine (process_sep; the 256 x 256 engine, stored waves), its 1D factors in LDS.
-template <int N, bool RC, bool SEP>
+template <int N, int WV, bool SEP>
 __global__ __launch_bounds__(Cfg<N>::NT, Cfg<N>::MINB) void k_batch(PosArgs A) {
     constexpr int NT = Cfg<N>::NT;
     MSET_LDS(N, NT)
@@ -665,7 +727,7 @@ __global__ __launch_bounds__(Cfg<N>::NT, Cfg<N>::MINB) void k_batch(PosArgs A) {
         for (int p = blockIdx.x; p < A.count; p += gridDim.x) {
             const int k = A.order ? A.order[A.first + p] : A.first + p;
             const int slab = p % A.nslab;
-            process_position<N, NT, K_BATCH, RC>(A, k, blockIdx.x, slab, p < A.nslab, E, phx, phy, red);
+            process_position<N, NT, K_BATCH, WV>(A, k, blockIdx.x, slab, p < A.nslab, E, phx, phy, red);
             __syncthreads();
         }
     }
@@ -784,10 +846,14 @@ hipError_t launch_plain(int n, const PosArgs& A, int k, hipStream_t st) {
 
 hipError_t launch_batch(int n, const PosArgs& A, int grid, hipStream_t st) {
     if (A.sep) {
-        if (n != 256 || A.recompute || A.grad_mode != GRAD_REF || A.sset) return hipErrorInvalidValue;
-        hipLaunchKernelGGL((k_batch<256, false, true>), dim3(grid), dim3(Cfg<256>::NT), 0, st, A);
-    } else if (A.recompute) MSET_DISPATCH(n, hipLaunchKernelGGL((k_batch<NN, true, false>), dim3(grid), dim3(Cfg<NN>::NT), 0, st, A))
-    else MSET_DISPATCH(n, hipLaunchKernelGGL((k_batch<NN, false, false>), dim3(grid), dim3(Cfg<NN>::NT), 0, st, A));
+        if (n != 256 || A.waves != WAVES_STORE || A.grad_mode != GRAD_REF || A.sset) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_batch<256, WAVES_STORE, true>), dim3(grid), dim3(Cfg<256>::NT), 0, st, A);
+    } else if (A.waves == WAVES_RECOMPUTE) {
+        MSET_DISPATCH(n, hipLaunchKernelGGL((k_batch<NN, WAVES_RECOMPUTE, false>), dim3(grid), dim3(Cfg<NN>::NT), 0, st, A))
+    } else if (A.waves == WAVES_CHECKPOINT) {
+        if (A.sset || A.ckseg < 2) return hipErrorInvalidValue;
+        MSET_DISPATCH(n, hipLaunchKernelGGL((k_batch<NN, WAVES_CHECKPOINT, false>), dim3(grid), dim3(Cfg<NN>::NT), 0, st, A))
+    } else MSET_DISPATCH(n, hipLaunchKernelGGL((k_batch<NN, WAVES_STORE, false>), dim3(grid), dim3(Cfg<NN>::NT), 0, st, A));
     return hipGetLastError();
 }
 
@@ -815,7 +881,7 @@ hipError_t launch_plane_fft(int n, const float2* in, float2* out, const float2* 
 }
 
 hipError_t occupancy_batch(int n, int* blocks_per_cu) {
-    MSET_DISPATCH(n, return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, (k_batch<NN, false, false>), Cfg<NN>::NT, 0));
+    MSET_DISPATCH(n, return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, (k_batch<NN, WAVES_STORE, false>), Cfg<NN>::NT, 0));
     return hipSuccess;
 }
 

@@ -1,4 +1,4 @@
-"""Stored versus recomputed exit waves at 256^3 (the memory case of the adjoint verification config).
+"""Stored, recomputed and checkpointed exit waves at 256^3 (the memory case of the adjoint verification config).
 
     python -m mset_amd.exitwaves [--positions 256] [--repeat 2]
 
@@ -6,8 +6,10 @@ Volume 256^3 at 0.4 A (FCC Gaussian atoms, 8 % occupancy), probe 256 x 256 at 30
 of one voxel (0.4 A) so 256 propagation planes, a 16 x 16 scan at 0.8 A with sub-pixel jitter, Poisson
 data simulated on the device.  For each variant the batched forward + gradient at the fixed state runs
 over every position; the line reports the per-slot and total scratch (waves and gradient slabs of the
-resident positions), the kernel time and the gradient difference between the variants.  Prints one
-JSON line.
+resident positions), the kernel time and the gradient difference between the variants.  At 256 x 256 the
+stored variant runs the separable-propagator routine; `store_general` is the same variant on the general
+routine, which the recomputing and checkpointing variants run, so that the checkpointed waves (the same
+forward steps again) can be compared with stored ones of the same arithmetic.  Prints one JSON line.
 """
 import argparse
 import json
@@ -38,12 +40,18 @@ def run(positions=256, repeat=2, seed=4, wl=None):
     out = dict(workload='256^3 volume at 0.4 A, probe 256^2, %d planes, %d positions, 1 tilt' % (wl.L, len(xl)),
                variants={})
     grads = {}
-    for name, mode in (('store', lib.EXIT_WAVES_STORE), ('recompute', lib.EXIT_WAVES_RECOMPUTE)):
+    planes = {lib.EXIT_WAVES_STORE: wl.L + 2, lib.EXIT_WAVES_RECOMPUTE: 5,
+              lib.EXIT_WAVES_CHECKPOINT: checkpoint_planes(wl.L)}
+    for name, mode, engine in (('store', lib.EXIT_WAVES_STORE, lib.ENGINE_AUTO),
+                               ('store_general', lib.EXIT_WAVES_STORE, lib.ENGINE_GENERAL),
+                               ('recompute', lib.EXIT_WAVES_RECOMPUTE, lib.ENGINE_AUTO),
+                               ('checkpoint', lib.EXIT_WAVES_CHECKPOINT, lib.ENGINE_AUTO)):
         ctx = lib.Context(0)
         try:
             ctx.upload(V0, wl.probe, wl.prop, xl, yl, M, wl.sigma, wl.dx, slice_binning=wl.binning,
                        n_scan=wl.n_scans[0])
             ctx.set_exit_waves(mode)
+            ctx.set_engine(engine)
             g = ctx.grad()                                  # sizes the scratch
             ctx.profile(True)
             for _ in range(repeat):
@@ -56,19 +64,32 @@ def run(positions=256, repeat=2, seed=4, wl=None):
         grads[name] = g
         out['variants'][name] = dict(
             resolved=resolved, slots=min(slots, len(xl)), scratch_bytes_per_slot=slot_bytes,
-            wave_bytes_per_slot=(5 if resolved == lib.EXIT_WAVES_RECOMPUTE else wl.L + 2) * wl.P * wl.P * 8,
+            wave_bytes_per_slot=planes[resolved] * wl.P * wl.P * 8,
             scratch_gb=min(slots, len(xl)) * slot_bytes / 1e9,
             kernel_ms=ms / max(n, 1), pos_slices_per_s=len(xl) * wl.L / (ms / max(n, 1) * 1e-3))
-    s, r = grads['store'], grads['recompute']
-    out['recompute_vs_store'] = dict(
-        dV_rel=float(np.linalg.norm(r[0] - s[0]) / np.linalg.norm(s[0])),
-        dprobe_rel=float(np.linalg.norm(r[1] - s[1]) / np.linalg.norm(s[1])),
-        loss_rel=float(abs(r[3] / s[3] - 1)))
-    vs, vr = out['variants']['store'], out['variants']['recompute']
+    def diff(r, s):
+        return dict(dV_rel=float(np.linalg.norm(r[0] - s[0]) / np.linalg.norm(s[0])),
+                    dprobe_rel=float(np.linalg.norm(r[1] - s[1]) / np.linalg.norm(s[1])),
+                    loss_rel=float(abs(r[3] / s[3] - 1)), dV_bitwise=bool(np.array_equal(r[0], s[0])))
+    out['recompute_vs_store'] = diff(grads['recompute'], grads['store'])
+    out['checkpoint_vs_store_general'] = diff(grads['checkpoint'], grads['store_general'])
+    vs, vr, vc = out['variants']['store'], out['variants']['recompute'], out['variants']['checkpoint']
     out['wave_memory_ratio'] = vs['wave_bytes_per_slot'] / vr['wave_bytes_per_slot']
     out['scratch_memory_ratio'] = vs['scratch_bytes_per_slot'] / vr['scratch_bytes_per_slot']
     out['time_ratio'] = vr['kernel_ms'] / vs['kernel_ms']
+    out['checkpoint_wave_memory_ratio'] = vs['wave_bytes_per_slot'] / vc['wave_bytes_per_slot']
+    out['checkpoint_time_ratio'] = vc['kernel_ms'] / vs['kernel_ms']
+    out['checkpoint_time_vs_store_general'] = vc['kernel_ms'] / out['variants']['store_general']['kernel_ms']
     return out
+
+
+def checkpoint_planes(L):
+    """Per-slot wave planes of the checkpoint variant: psi0, Fourier psi0, L // K checkpoints, the K - 1 other
+    waves of a segment and the parked adjoint wave, K = ceil(sqrt(L + 1))."""
+    k = 2
+    while k * k < L + 1:
+        k += 1
+    return L // k + k + 2
 
 
 def main():

@@ -30,7 +30,7 @@ EXPORTS = ['mset_create', 'mset_destroy', 'mset_last_error', 'mset_stream', 'mse
            'mset_recon_set_slot_limit', 'mset_host_alloc', 'mset_host_free', 'mset_grad_noalloc', 'mset_set_engine',
            'mset_recon_set_engine']
 PATH_PLAIN, PATH_BATCHED = 0, 1
-EXIT_WAVES_AUTO, EXIT_WAVES_STORE, EXIT_WAVES_RECOMPUTE = 0, 1, 2
+EXIT_WAVES_AUTO, EXIT_WAVES_STORE, EXIT_WAVES_RECOMPUTE, EXIT_WAVES_CHECKPOINT = 0, 1, 2, 3
 ENGINE_AUTO, ENGINE_GENERAL = 0, 1
 
 
@@ -306,7 +306,7 @@ class Context:
         return out, float(e[0])
 
     def set_exit_waves(self, mode):
-        """EXIT_WAVES_AUTO / _STORE / _RECOMPUTE for the batched calls (mset_set_exit_waves)."""
+        """EXIT_WAVES_AUTO / _STORE / _RECOMPUTE / _CHECKPOINT for the batched calls (mset_set_exit_waves)."""
         _chk(self._L.mset_set_exit_waves(self._h, int(mode)))
 
     def set_slot_limit(self, slots):

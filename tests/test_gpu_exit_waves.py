@@ -1,6 +1,8 @@
-"""Recomputed exit waves (mset_set_exit_waves): the batched adjoint that rebuilds psi_a by inverting the
-forward sweep gives the stored-wave gradient and the FP64 oracle's, with 5 scratch planes per resident
-position instead of L + 2.
+"""Recomputed and checkpointed exit waves (mset_set_exit_waves).  The batched adjoint that rebuilds psi_a by
+inverting the forward sweep gives the stored-wave gradient and the FP64 oracle's, with 5 scratch planes per
+resident position instead of L + 2.  The checkpointed variant keeps every K-th wave and one segment and
+rebuilds the other segments with the forward sweep's own steps: the stored variant's waves, for any
+propagator.
 
 Tolerances: the forward pass is the same arithmetic in both variants (the two kernel instantiations
 may order a few roundings differently), so the losses agree to 1e-6; the rebuilt waves carry the FP32
@@ -19,6 +21,19 @@ pytestmark = pytest.mark.gpu
 
 REBUILD_TOL = 1e-4
 GRAD_TOL = 1e-3
+# checkpointed vs stored waves: the same forward steps, in another kernel instantiation.  Some engines give
+# the stored variant's gradient bit for bit (the N = 32 and 64 LDS engines); elsewhere the compiler contracts
+# a complex product into FMAs differently in the rebuild than in the forward sweep (up to 3.5e-6 after 8-18
+# planes at N = 48 and 128), so the bound is a few ulps per plane
+CKPT_TOL = 1e-5
+
+
+def ck_planes(L):
+    """per-slot wave planes of the checkpoint variant, K = ceil(sqrt(L + 1))"""
+    k = 2
+    while k * k < L + 1:
+        k += 1
+    return L // k + k + 2
 
 
 def _upload(ctx, c, mode, prop=None, steps=(0.0, 0.0, 0.0)):
@@ -63,6 +78,65 @@ def test_recomputed_waves_give_the_stored_gradient(n, nz, binning, mode):
     assert rel(rpos, odpos) < GRAD_TOL
 
 
+@pytest.mark.parametrize('n,nz,binning,mode', [(64, 16, 2, O.MODE_REF), (64, 12, 1, O.MODE_REF),
+                                               (64, 12, 1, O.MODE_EXACT), (32, 40, 1, O.MODE_EXACT),
+                                               (48, 15, 1, O.MODE_REF), (128, 16, 2, O.MODE_EXACT),
+                                               (128, 36, 2, O.MODE_REF), (256, 12, 2, O.MODE_REF),
+                                               (256, 10, 1, O.MODE_EXACT)])
+def test_checkpointed_waves_give_the_stored_gradient(n, nz, binning, mode):
+    """Segments that end on the last plane (L a multiple of K) and ones that do not, both pairings, every
+    engine; at 256 the stored variant is run on the general routine, which the checkpointed one uses."""
+    from mset_amd import lib
+    c = small_case(n=n, nz=nz, binning=binning, npos=6, seed=7 * n + nz, spread=n / 8)
+    L = -(-nz // binning)
+    print('case: checkpointed exit waves', n, nz, binning, mode, 'L', L, flush=True)
+    ctx = lib.Context(0)
+    try:
+        _upload(ctx, c, mode)
+        ctx.set_engine(lib.ENGINE_GENERAL)
+        ctx.set_exit_waves(lib.EXIT_WAVES_STORE)
+        dV, dp, dpos, err = ctx.grad()
+        ctx.set_exit_waves(lib.EXIT_WAVES_CHECKPOINT)
+        cV, cp, cpos, cerr = ctx.grad()
+        mc, bc = ctx.exit_waves()
+    finally:
+        ctx.close()
+    assert mc == lib.EXIT_WAVES_CHECKPOINT and bc == n * n * (ck_planes(L) * 8 + L * 4)
+    print('checkpointed vs stored: dV %.2e dprobe %.2e dpos %.2e loss %.2e bitwise dV %s'
+          % (rel(cV, dV), rel(cp, dp), rel(cpos, dpos), abs(cerr / err - 1), np.array_equal(cV, dV)), flush=True)
+    assert abs(cerr / err - 1) < CKPT_TOL
+    assert rel(cV, dV) < CKPT_TOL and rel(cp, dp) < CKPT_TOL
+    assert np.max(np.abs(cpos - dpos)) <= CKPT_TOL * np.max(np.abs(dpos))
+    odV, odp, odpos, _ = O.grad_fixed(c['V'], c['probe'], c['prop'], c['bprop'], c['xl'], c['yl'], c['M'],
+                                      c['bin'], c['sig'], c['dx'], mode=mode)
+    assert rel(cV, odV) < GRAD_TOL and rel(cp, odp) < GRAD_TOL and rel(cpos, odpos) < GRAD_TOL
+
+
+def test_checkpoints_serve_a_propagator_that_recompute_refuses():
+    """A non-unit-modulus propagator (0.9 of the Fresnel one): the inverted step is refused, the checkpointed
+    variant gives the stored-wave gradient."""
+    from mset_amd import lib
+    c = small_case(n=64, nz=24, binning=1, npos=5, seed=21, spread=6.0)
+    prop = 0.9 * c['prop']
+    ctx = lib.Context(0)
+    try:
+        _upload(ctx, c, O.MODE_REF, prop=prop)
+        ctx.set_exit_waves(lib.EXIT_WAVES_RECOMPUTE)
+        with pytest.raises(lib.MsetError) as e:
+            ctx.grad()
+        assert e.value.code == lib.MSET_EINVAL
+        ctx.set_exit_waves(lib.EXIT_WAVES_STORE)
+        dV, dp, dpos, err = ctx.grad()
+        ctx.set_exit_waves(lib.EXIT_WAVES_CHECKPOINT)
+        cV, cp, cpos, cerr = ctx.grad()
+        assert ctx.exit_waves()[0] == lib.EXIT_WAVES_CHECKPOINT
+    finally:
+        ctx.close()
+    print('non-unit propagator, checkpointed vs stored: dV %.2e dprobe %.2e' % (rel(cV, dV), rel(cp, dp)), flush=True)
+    assert abs(cerr / err - 1) < CKPT_TOL and rel(cV, dV) < CKPT_TOL and rel(cp, dp) < CKPT_TOL
+    assert np.max(np.abs(cpos - dpos)) <= CKPT_TOL * np.max(np.abs(dpos))
+
+
 def test_recompute_needs_a_unit_modulus_propagator_and_auto_stores():
     from mset_amd import lib
     c = small_case(n=32, nz=12, binning=1, npos=3, seed=5, spread=4.0)
@@ -85,12 +159,12 @@ def test_recompute_needs_a_unit_modulus_propagator_and_auto_stores():
 
 
 def test_recon_pass_with_recomputed_waves():
-    """A graph-captured batched update pass of a two-tilt series: rebuilt waves give the stored-wave
-    volume, and switching the variant drops the captured graph."""
+    """A graph-captured batched update pass of a two-tilt series: rebuilt and checkpointed waves give the
+    stored-wave volume, and switching the variant drops the captured graph."""
     from mset_amd import lib
     c = small_case(n=64, nz=16, binning=2, npos=8, seed=11, spread=6.0)
     out = {}
-    for mode in (lib.EXIT_WAVES_STORE, lib.EXIT_WAVES_RECOMPUTE):
+    for mode in (lib.EXIT_WAVES_STORE, lib.EXIT_WAVES_RECOMPUTE, lib.EXIT_WAVES_CHECKPOINT):
         r = lib.Recon(c['V'], c['prop'], 2, c['sig'], c['dx'], slice_binning=c['bin'], step_size=(50.0, 0.0, 0.0))
         try:
             r.set_exit_waves(mode)
@@ -101,17 +175,18 @@ def test_recon_pass_with_recomputed_waves():
             out[mode] = r.get()[0]
         finally:
             r.close()
-    d = out[lib.EXIT_WAVES_RECOMPUTE] - c['V']
     ds = out[lib.EXIT_WAVES_STORE] - c['V']
-    assert np.linalg.norm(d - ds) <= REBUILD_TOL * np.linalg.norm(ds) + 1e-6 * np.linalg.norm(c['V'])
+    for mode, tol in ((lib.EXIT_WAVES_RECOMPUTE, REBUILD_TOL), (lib.EXIT_WAVES_CHECKPOINT, CKPT_TOL)):
+        d = out[mode] - c['V']
+        assert np.linalg.norm(d - ds) <= tol * np.linalg.norm(ds) + 1e-6 * np.linalg.norm(c['V']), mode
 
 
 @pytest.mark.parametrize('n,nz,binning,mode', [(128, 16, 2, O.MODE_REF), (128, 12, 3, O.MODE_EXACT),
                                                (256, 8, 2, O.MODE_REF), (256, 6, 1, O.MODE_EXACT)])
 def test_slots_reused_by_several_positions_stored_and_recomputed(n, nz, binning, mode):
     """Two resident slots for eight positions: each slot runs four positions in one launch, so its later
-    positions add to the slabs its first one wrote, and its wave and plane scratch are reused.  Both exit-wave
-    variants against the FP64 oracle, and the per-position terms against the unlimited grid (bitwise: they do
+    positions add to the slabs its first one wrote, and its wave and plane scratch are reused.  The three
+    exit-wave variants against the FP64 oracle, and the per-position terms against the unlimited grid (bitwise: they do
     not depend on which slot ran the position)."""
     from mset_amd import lib
     c = small_case(n=n, nz=nz, binning=binning, npos=8, seed=5 * n + nz, spread=n / 8)
@@ -122,7 +197,7 @@ def test_slots_reused_by_several_positions_stored_and_recomputed(n, nz, binning,
     try:
         _upload(ctx, c, mode)
         full = ctx.grad()
-        for ew in (lib.EXIT_WAVES_STORE, lib.EXIT_WAVES_RECOMPUTE):
+        for ew in (lib.EXIT_WAVES_STORE, lib.EXIT_WAVES_RECOMPUTE, lib.EXIT_WAVES_CHECKPOINT):
             ctx.set_exit_waves(ew)
             ctx.set_slot_limit(2)
             dV, dp, dpos, err = ctx.grad()
