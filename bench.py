@@ -149,11 +149,14 @@ def main():
     if not args.no_autotune:
         from mset_amd import autotune
         full = lib.batch_slots(wl.P)
-        tuned = autotune.tune_recon(r, batches=batches, slots=[full, full // 2], log=log)
+        tuned = autotune.tune_recon(r, batches=batches, slots=[0, full // 2], log=log)
         args.batch = tuned['batch']
-        tuned = dict(engine=['auto (separable at 256)', 'general'][tuned['engine']], batch=tuned['batch'],
-                     slots=tuned['slots'],
-                     pass_ms={'engine %d / slots %d / batch %d' % (e, sl, b): round(dt, 2)
+
+        def slots_name(sl):
+            return 'all resident' if sl == 0 else str(sl)
+        tuned = dict(engine=autotune.engine_name(tuned['engine']), batch=tuned['batch'],
+                     slots=slots_name(tuned['slots']),
+                     pass_ms={'%s / slots %s / batch %d' % (autotune.engine_name(e), slots_name(sl), b): round(dt, 2)
                               for e, sl, b, dt, _ in tuned['table']})
     else:
         args.batch = batches[0]

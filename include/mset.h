@@ -149,14 +149,18 @@ int mset_exit_waves(mset_ctx* ctx, int* resolved, size_t* bytes_per_slot);
  * 0 (the default) = every workgroup the device keeps resident (mset_batch_slots).  A tuning knob with
  * no counterpart in the reference: fewer slots hold less scratch and may keep more of it in cache. */
 int mset_set_slot_limit(mset_ctx* ctx, int slots);
-/* Plane routine of the 256 x 256 kernels.  AUTO (the default): when the propagator factors as
- * p(kx, ky) = px(kx) py(ky) (the Fresnel propagator of Func_generate_propagator.m does, checked at upload)
- * the forward kernels and the reference-pairing batched gradient with stored waves run the separable
- * routine, which applies the two 1D half-steps of neighbouring planes in one pass over the plane (half the
- * passes); GENERAL always runs the 2D routine.  Both agree with the FP64 oracle to the same tolerances; a
- * tuning knob with no counterpart in the reference. */
+/* Engine flags of the position kernels, a tuning knob with no counterpart in the reference; every choice
+ * agrees with the FP64 oracle to the same tolerances.  AUTO (0, the default): at 256 x 256, when the
+ * propagator factors as p(kx, ky) = px(kx) py(ky) (the Fresnel propagator of Func_generate_propagator.m
+ * does, checked at upload), the forward kernels and the reference-pairing batched gradient with stored
+ * waves run the separable routine, which applies the two 1D half-steps of neighbouring planes in one pass
+ * over the plane (half the passes).  GENERAL: always the 2D routine.  ALT_SHAPE: the batched kernel with
+ * stored waves runs the alternative workgroup shape of its plane size: twice the threads per position of
+ * the LDS engine up to 64, 1024 threads at 96, the LDS engine with 1024 threads instead of the register engine at
+ * 128, 16-line tiles with 256 threads (three workgroups per CU instead of two) at 256.  The flags combine. */
 #define MSET_ENGINE_AUTO 0
 #define MSET_ENGINE_GENERAL 1
+#define MSET_ENGINE_ALT_SHAPE 2
 int mset_set_engine(mset_ctx* ctx, int engine);
 
 /* Mini-batch variant of 'run_mset': positions are taken in order in batches of `batch`; each batch's

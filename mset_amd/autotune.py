@@ -1,18 +1,21 @@
-"""Autotuner of the batched path: picks the persistent grid and the mini-batch size by timing them on the
-workload itself.
+"""Autotuner of the batched path: picks the engine, the persistent grid and the mini-batch size by timing
+them on the workload itself.
 
 Tunables (both leave the kernels' arithmetic unchanged):
-  slots  the persistent grid of the batched kernels (mset_set_slot_limit).  Fewer resident positions
-         hold less scratch (saved waves and, at 256 x 256, the plane buffers) and keep more of it in L2 /
-         MALL; more of them hide memory latency better.  Gradient slabs are assigned by position, not by
-         workgroup, and a capped grid divides the slab count, so the sums are bitwise the same for every
-         candidate: tuning never changes a result.
+  slots  the persistent grid of the batched kernels (mset_set_slot_limit; 0 = every resident workgroup).
+         Fewer resident positions hold less scratch (saved waves and, at 256 x 256, the plane buffers) and
+         keep more of it in L2 / MALL; more of them hide memory latency better.  Gradient slabs are
+         assigned by position, not by workgroup, and a capped grid divides the slab count, so for one
+         engine the sums are bitwise the same for every grid: tuning the grid never changes a result.
   batch  positions per launch of an update pass (the mini-batch size), from the range the caller
          allows: it sets how many launches, reductions and updates a pass makes and how well the last
          wave of workgroups fills the device.
-  engine the plane routine at 256 x 256 (mset_set_engine): the separable-propagator routine, one pass over
-         the plane per propagation step, or the general 2D routine, two passes per step.
-The radix plans and LDS tiles of the routines are fixed per plane size at compile time.
+  engine mset_set_engine flags: the plane routine at 256 x 256 (the separable-propagator routine, one pass
+         over the plane per propagation step, or the general 2D routine, two passes per step) and the
+         workgroup shape of the batched kernel at every size (the default shape, or the alternative one:
+         threads per position of the LDS engine, the LDS engine instead of the register engine at 128,
+         16- instead of 32-line tiles at 256).  Engines sum in different orders: results agree to FP32
+         rounding.
 
 Each candidate runs one update pass from a snapshot of the tilt series, which is restored afterwards,
 so tuning leaves the reconstruction unchanged.  The kernels are timed with HIP events on the series'
@@ -29,20 +32,30 @@ _CACHE = {}
 
 
 def slot_candidates(n, fracs=(1.0, 0.5, 0.25), device=0):
-    """Grid sizes to try: fractions of the resident workgroups that divide their count (a capped grid
-    must divide the slab count, which is the resident workgroups')."""
+    """Grid sizes to try: 0 (every resident workgroup of the engine's shape), then fractions of the default
+    shape's resident workgroups (a capped grid is reduced to a divisor of the slab count)."""
     full = lib.batch_slots(n, device)
     out = []
     for f in fracs:
-        s = max(1, int(round(full * f)))
+        s = 0 if f >= 1.0 else max(1, int(round(full * f)))
         if s not in out:
             out.append(s)
     return out
 
 
 def engine_candidates(n):
-    """Plane routines worth timing at plane size n (two exist at 256)."""
-    return (lib.ENGINE_AUTO, lib.ENGINE_GENERAL) if n == 256 else (lib.ENGINE_AUTO,)
+    """Engines worth timing at plane size n: the default and the alternative workgroup shape, and at 256 the
+    general routine (the separable one is the default)."""
+    if n == 256:
+        return (lib.ENGINE_AUTO, lib.ENGINE_ALT_SHAPE, lib.ENGINE_GENERAL)
+    return (lib.ENGINE_AUTO, lib.ENGINE_ALT_SHAPE)
+
+
+def engine_name(e):
+    """readable name of mset_set_engine flags"""
+    parts = ['general routine' if e & lib.ENGINE_GENERAL else 'default routine',
+             'alternative shape' if e & lib.ENGINE_ALT_SHAPE else 'default shape']
+    return ', '.join(parts)
 
 
 def _key(recon, batches, slots, engines):

@@ -87,7 +87,8 @@ struct Workspace {
     DevBuf<float> gslab, G;
     int exit_waves = MSET_EXIT_WAVES_AUTO;   // requested exit-wave variant of the batched kernels
     int slot_limit = 0;                      // cap on the persistent grid of the batched kernels (0: none)
-    int engine = MSET_ENGINE_AUTO;           // plane routine of the 256 x 256 kernels
+    int engine = MSET_ENGINE_AUTO;           // MSET_ENGINE_* flags: plane routine at 256, workgroup shape
+    int shape = 0;                           // workgroup shape of the last batched call (Cfg<N, shape>)
     int waves = WAVES_STORE;                 // the variant of the last batched call (WAVES_*)
     size_t slot_bytes = 0;                   // its per-slot scratch (waves + gradient slab)
     RotWork rot;
@@ -276,30 +277,39 @@ PosArgs base_args(mset_ctx* c) {
 // The separable-propagator kernels (the 256 x 256 engine) serve the forward-only launches and the batched
 // gradient with the reference pairing and stored waves; everything else runs the general kernels.
 bool sep_fwd(const mset_ctx* c) {
-    return plane_in_global(c->n) && c->prop_sep && !sset(c) && c->ws->engine != MSET_ENGINE_GENERAL;
+    return plane_in_global(c->n) && c->prop_sep && !sset(c) && !(c->ws->engine & MSET_ENGINE_GENERAL);
 }
 bool sep_batch(const mset_ctx* c) { return sep_fwd(c) && c->p.grad_mode == GRAD_REF && c->ws->waves == WAVES_STORE; }
 
 // workgroups of the batched kernels the device keeps resident at once (1 per CU at N = 128)
-int device_slots(mset_ctx* c) {
+// workgroup shape of a batched call: the alternative shape (MSET_ENGINE_ALT_SHAPE) exists for stored waves
+int shape_for(const mset_ctx* c, int waves) {
+    return (c->ws->engine & MSET_ENGINE_ALT_SHAPE) && waves == WAVES_STORE ? 1 : 0;
+}
+
+// resident workgroups of the batched kernel of a workgroup shape (by default, the last batched call's)
+int device_slots(mset_ctx* c, int shape) {
     int per_cu = 1;
-    if (occupancy_batch(c->n, &per_cu) != hipSuccess || per_cu < 1) per_cu = 1;
+    if (occupancy_batch(c->n, shape, &per_cu) != hipSuccess || per_cu < 1) per_cu = 1;
     return c->cus * per_cu;
 }
+int device_slots(mset_ctx* c) { return device_slots(c, c->ws->shape); }
 
 // the persistent grid of the batched kernels: the resident workgroups, capped by the slot limit and the
 // positions
-int batch_slots(mset_ctx* c) {
-    int s = device_slots(c);
+int batch_slots(mset_ctx* c, int shape) {
+    int s = device_slots(c, shape);
     if (c->ws->slot_limit > 0 && s > c->ws->slot_limit) s = c->ws->slot_limit;
     if (s > c->npos) s = c->npos;
     return s < 1 ? 1 : s;
 }
+int batch_slots(mset_ctx* c) { return batch_slots(c, c->ws->shape); }
 
 // Gradient slabs of the context's batched launches: position p of a launch adds to slab p % S.  S depends
-// on the device and the positions only, never on the slot limit, so capping the grid leaves every sum
-// (and so every result) bitwise unchanged.
-int grad_slabs(mset_ctx* c) { return std::max(1, std::min(device_slots(c), c->npos)); }
+// on the device, the workgroup shape and the positions only, never on the slot limit, so capping the grid
+// leaves every sum (and so every result) bitwise unchanged.
+int grad_slabs(mset_ctx* c, int shape) { return std::max(1, std::min(device_slots(c, shape), c->npos)); }
+int grad_slabs(mset_ctx* c) { return grad_slabs(c, c->ws->shape); }
 
 // grid of a batched launch of `count` positions on S slabs: at most `want` workgroups and, when slabs take
 // several positions (count > S), a divisor of S, so that each slab's positions run on one workgroup in order
@@ -359,13 +369,14 @@ int resolve_exit_waves(mset_ctx* c, int slots, int* waves) {
 
 int ensure_batch_buffers(mset_ctx* c) {
     Workspace& W = *c->ws;
+    int wv = WAVES_STORE;
+    CHK(resolve_exit_waves(c, batch_slots(c, 0), &wv));
+    W.waves = wv;
+    W.shape = shape_for(c, wv);
     const int s = batch_slots(c);
     CHK(ensure_fixed_buffers(c));
-    int wv = WAVES_STORE;
-    CHK(resolve_exit_waves(c, s, &wv));
     const int save_planes = wave_planes(c, wv);
     CHK(W.save.ensure((size_t)s * save_planes * plane(c), "save waves"));
-    W.waves = wv;
     W.slot_bytes = plane(c) * (save_planes * sizeof(float2) + (size_t)planes(c) * sizeof(float));
     const int S = grad_slabs(c);
     CHK(W.gslab.ensure((size_t)S * c->L * plane(c), "gradient slabs"));
@@ -459,7 +470,7 @@ int issue_grad(mset_ctx* c, int first, int count, bool apply, int want) {
     // every used slab receives at least one position, and its first one writes it
     {
         Timed t(c, "position_batch");
-        HIPCHK(launch_batch(c->n, A, grid, c->stream));
+        HIPCHK(launch_batch(c->n, A, grid, W.shape, c->stream));
     }
     HIPCHK(launch_reduce_slabs(W.gslab.p, used, (size_t)planes(c) * plane(c), W.G.p, c->stream));
     if (sep) HIPCHK(launch_untranspose_odd(W.G.p, c->n, c->L, c->stream));
@@ -828,8 +839,9 @@ int mset_grad_noalloc(mset_ctx* c, int first, int count, float* dV, float* dprob
     auto need = [&](int S, int G) { return fixed + (size_t)S * (s_g + s_ps) + (size_t)G * (s_save + s_pl); };
     // mset_grad's slabs and grid when they fit (then the sums are bitwise mset_grad's), else as many resident
     // positions as fit, one slab each
-    int S = std::min(grad_slabs(c), count);
-    int G = launch_grid(batch_slots(c), count, S);
+    const int shape = shape_for(c, WAVES_STORE);
+    int S = std::min(grad_slabs(c, shape), count);
+    int G = launch_grid(batch_slots(c, shape), count, S);
     while (G > 1 && need(S, G) > MSET_NOALLOC_SCRATCH_BYTES) G = launch_grid(G - 1, count, S);
     if (need(S, G) > MSET_NOALLOC_SCRATCH_BYTES) {
         S = G = (int)std::min<size_t>((size_t)count, (MSET_NOALLOC_SCRATCH_BYTES - std::min(fixed, (size_t)MSET_NOALLOC_SCRATCH_BYTES)) /
@@ -871,7 +883,7 @@ int mset_grad_noalloc(mset_ctx* c, int first, int count, float* dV, float* dprob
     A.sep = sep ? 1 : 0;
     A.tph = (const float*)trans;
     A.tph_b = (const float*)trans + L * n2;
-    HIPCHK(launch_batch(c->n, A, G, c->stream));
+    HIPCHK(launch_batch(c->n, A, G, shape, c->stream));
     HIPCHK(launch_reduce_slabs(gslab, S, L * n2, dV, c->stream));
     if (sep) HIPCHK(launch_untranspose_odd(dV, c->n, c->L, c->stream));
     HIPCHK(launch_reduce_cslabs(pslab, S, n2, pslab, c->stream));
@@ -893,7 +905,7 @@ int mset_waves(mset_ctx* c, int k, float* waves_out) {
     A.want = 0;
     A.waves = WAVES_STORE;
     A.nslab = 1;
-    HIPCHK(launch_batch(c->n, A, 1, c->stream));
+    HIPCHK(launch_batch(c->n, A, 1, 0, c->stream));
     HIPCHK(hipMemcpyAsync(waves_out, c->ws->save.p, sizeof(float2) * (size_t)(c->L + 1) * plane(c),
                           hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -933,7 +945,7 @@ int mset_set_slot_limit(mset_ctx* c, int slots) {
 
 int mset_set_engine(mset_ctx* c, int engine) {
     CHK(check_ctx(c));
-    if (engine != MSET_ENGINE_AUTO && engine != MSET_ENGINE_GENERAL) return fail(MSET_EINVAL, "bad engine");
+    if (engine & ~(MSET_ENGINE_GENERAL | MSET_ENGINE_ALT_SHAPE)) return fail(MSET_EINVAL, "bad engine");
     c->ws->engine = engine;
     return MSET_OK;
 }
@@ -1006,7 +1018,7 @@ int mset_batch_slots(int device, int n, int* slots) {
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, device));
     int per_cu = 1;
-    HIPCHK(occupancy_batch(n, &per_cu));
+    HIPCHK(occupancy_batch(n, 0, &per_cu));
     *slots = prop.multiProcessorCount * (per_cu < 1 ? 1 : per_cu);
     return MSET_OK;
 }
@@ -1195,7 +1207,7 @@ int mset_recon_error_pass(mset_recon* r, double* errs) {
     CHK(r->err_all.ensure(tot, "error-pass errors"));
     CHK(r->items.ensure(tot, "error-pass work items"));
     CHK(r->tilt_tab.ensure(T, "error-pass tilt table"));
-    int grid = std::max(1, device_slots(c0));
+    int grid = std::max(1, device_slots(c0, 0));
     if (r->ws->slot_limit > 0) grid = std::min(grid, r->ws->slot_limit);
     grid = (int)std::min<size_t>((size_t)grid, tot);
     if (plane_in_global(r->n)) {
@@ -1309,7 +1321,7 @@ int mset_recon_set_slot_limit(mset_recon* r, int slots) {
 
 int mset_recon_set_engine(mset_recon* r, int engine) {
     if (!r) return fail(MSET_EINVAL, "null reconstruction");
-    if (engine != MSET_ENGINE_AUTO && engine != MSET_ENGINE_GENERAL) return fail(MSET_EINVAL, "bad engine");
+    if (engine & ~(MSET_ENGINE_GENERAL | MSET_ENGINE_ALT_SHAPE)) return fail(MSET_EINVAL, "bad engine");
     HIPCHK(hipSetDevice(r->device));
     if (engine != r->ws->engine) {
         HIPCHK(hipStreamSynchronize(r->stream));

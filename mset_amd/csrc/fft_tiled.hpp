@@ -1,15 +1,16 @@
 // fft_tiled.hpp -- 2D FFT of a 256 x 256 complex plane that lives in global memory (CDNA4, gfx950).
 //
 // A 256 x 256 complex plane (512 KB) fits neither one CU's LDS nor its registers, so the plane stays in
-// two per-workgroup global buffers and every 1D pass streams it through LDS in tiles of TL = 32 lines:
+// two per-workgroup global buffers and every 1D pass streams it through LDS in tiles of TL lines (TL = NT / 16:
+// 32 with 512 threads, 16 with 256):
 //   RS  row-major [y][x]     the state after an inverse transform (C orientation, lines are columns x)
 //   XS  transposed [x][y]    the state after a forward transform  (R orientation, lines are rows y)
 // A 256-point line transform is two radix-16 Stockham stages, one butterfly per thread per stage
-// (32 lines x 16 butterflies = 512 threads), exchanging through LDS.  A pass loads a tile from global
+// (TL lines x 16 butterflies = NT threads), exchanging through LDS.  A pass loads a tile from global
 // memory (applying `pre`), runs up to two line transforms with an element functor between them, and
 // stores the tile back in place or, when the pass changes the layout, through LDS with the lanes along
 // the other axis so that every global access stays coalesced.  The LDS tile is float2 s[pos][line] with
-// pitch TL + 1 (odd): lanes along lines touch consecutive words, lanes along positions stride by 33
+// pitch TL + 1 (odd): lanes along lines touch consecutive words, lanes along positions stride by TL + 1
 // words, both free of bank conflicts.
 //
 //   fwd: column pass RS -> XS (pre fused on load), row pass XS -> XS (post fused)
@@ -78,12 +79,12 @@ template <class A, class B, class C> struct HasTileStep<Seq<A, B, C>> {
 
 template <int N, int NT_> struct TiledEngine {
     static constexpr int NT = NT_;
-    static constexpr int TL = 32;                // lines per tile
     static constexpr int R = 16;                 // radix of both stages (N = 256)
+    static constexpr int TL = NT / (N / R);      // lines per tile
     static constexpr int TP = TL + 1;            // LDS pitch
     static constexpr int LDS_ELEMS = N * TP;
     static constexpr bool FUSED = true;          // provides the fused passes below
-    static_assert(N == 256 && NT == TL * (N / R), "one butterfly per thread per stage");
+    static_assert(N == 256 && NT == TL * (N / R) && (TL == 16 || TL == 32), "one butterfly per thread per stage");
     enum { NONE = 0, FWD = 1, INV = 2 };
     float2* s;
     const float2* tw;
@@ -120,7 +121,7 @@ template <int N, int NT_> struct TiledEngine {
         DFT<R, D == INV>::run(v);
     }
 
-    // One pass over the plane in tiles of 32 lines: load (through `pre`; LOADG: pre supplies the value),
+    // One pass over the plane in tiles of TL lines: load (through `pre`; LOADG: pre supplies the value),
     // transform D1, `mid` on every element, transform D2, store.  COL: lines are columns x, positions y,
     // the input is RS; otherwise lines are rows y, positions x, the input is XS.  SWAP: the output goes to
     // the other buffer (through LDS, lanes along its fast axis); otherwise back in place.  Points carry

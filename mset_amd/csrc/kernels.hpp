@@ -79,14 +79,15 @@ __host__ __device__ inline int checkpoint_planes(int L, int K) { return L / K + 
 bool plane_size_supported(int n);
 int position_threads(int n);
 hipError_t launch_plain(int n, const PosArgs& A, int k, hipStream_t st);
-hipError_t launch_batch(int n, const PosArgs& A, int grid, hipStream_t st);
+// shape: the workgroup shape of the batched kernel (0, or 1 = the alternative shape, stored waves only)
+hipError_t launch_batch(int n, const PosArgs& A, int grid, int shape, hipStream_t st);
 hipError_t launch_fwd(int n, const PosArgs& A, int grid, hipStream_t st);
 // forward + loss over the work items A.order[0 .. A.count) of several tilts (A.tilts)
 hipError_t launch_fwd_tilts(int n, const PosArgs& A, int grid, hipStream_t st);
 hipError_t launch_plane_fft(int n, const float2* in, float2* out, const float2* tw, int dir, float alpha, float beta,
                             float2* planes, hipStream_t st);
 bool plane_in_global(int n);     // the position kernels keep the plane in global memory (planes buffer)
-hipError_t occupancy_batch(int n, int* blocks_per_cu);
+hipError_t occupancy_batch(int n, int shape, int* blocks_per_cu);
 
 // element-wise kernels (elementwise.hip)
 hipError_t launch_ingest_patterns(const float* M, float* sqrtM, int n, long long npos, int transpose, hipStream_t st);

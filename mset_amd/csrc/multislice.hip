@@ -35,7 +35,12 @@
 namespace mset {
 
 // plane engine and workgroup size per plane size: register-resident at 128, LDS stages elsewhere
-template <int N> struct Cfg;   // NT threads; MINB: waves per SIMD the register budget must allow (HIP launch bounds)
+// Engine and workgroup shape per plane size.  NT threads; MINB: waves per SIMD the register budget must allow
+// (HIP launch bounds).  SH = 0: the shape every kernel uses; SH = 1: the alternative shape of the batched
+// kernel, a candidate for the autotuner (MSET_ENGINE_ALT_SHAPE): twice the threads per position of the LDS
+// engine up to N = 64, 1024 threads at 96, the LDS engine with 1024 threads instead of the register engine at 128,
+// and tiles of 16 lines with 256 threads (three workgroups per CU instead of two) at 256.
+template <int N, int SH = 0> struct Cfg;
 template <> struct Cfg<16> { static constexpr int NT = 64, MINB = 1;  using Eng = LdsEngine<16, 64>; };
 template <> struct Cfg<24> { static constexpr int NT = 64, MINB = 1;  using Eng = LdsEngine<24, 64>; };
 template <> struct Cfg<32> { static constexpr int NT = 64, MINB = 1;  using Eng = LdsEngine<32, 64>; };
@@ -44,6 +49,14 @@ template <> struct Cfg<64> { static constexpr int NT = 256, MINB = 1; using Eng 
 template <> struct Cfg<96> { static constexpr int NT = 576, MINB = 1; using Eng = LdsEngine<96, 576>; };
 template <> struct Cfg<128> { static constexpr int NT = 512, MINB = 1; using Eng = RegEngine<128>; };
 template <> struct Cfg<256> { static constexpr int NT = 512, MINB = 4; using Eng = TiledEngine<256, 512>; };
+template <> struct Cfg<16, 1> { static constexpr int NT = 128, MINB = 1; using Eng = LdsEngine<16, 128>; };
+template <> struct Cfg<24, 1> { static constexpr int NT = 128, MINB = 1; using Eng = LdsEngine<24, 128>; };
+template <> struct Cfg<32, 1> { static constexpr int NT = 128, MINB = 1; using Eng = LdsEngine<32, 128>; };
+template <> struct Cfg<48, 1> { static constexpr int NT = 384, MINB = 1; using Eng = LdsEngine<48, 384>; };
+template <> struct Cfg<64, 1> { static constexpr int NT = 512, MINB = 1; using Eng = LdsEngine<64, 512>; };
+template <> struct Cfg<96, 1> { static constexpr int NT = 1024, MINB = 1; using Eng = LdsEngine<96, 1024>; };
+template <> struct Cfg<128, 1> { static constexpr int NT = 1024, MINB = 1; using Eng = LdsEngine<128, 1024>; };
+template <> struct Cfg<256, 1> { static constexpr int NT = 256, MINB = 3; using Eng = TiledEngine<256, 256>; };
 
 
 enum { K_PLAIN = 0, K_BATCH = 1, K_FWD = 2 };
@@ -689,8 +702,8 @@ template <int N, int NT> __device__ __forceinline__ void load_twiddles(float2* t
     __syncthreads();
 }
 
-#define MSET_LDS(N, NT)                                          \
-    using Eng = typename Cfg<N>::Eng;                            \
+#define MSET_LDS(N, NT, SH)                                      \
+    using Eng = typename Cfg<N, SH>::Eng;                        \
     __shared__ float2 s[Eng::LDS_ELEMS];                         \
     __shared__ float2 tw[N], phx[N], phy[N];                     \
     __shared__ double red[(NT + 63) / 64];                       \
@@ -701,7 +714,7 @@ template <int N, int NT> __device__ __forceinline__ void load_twiddles(float2* t
 template <int N>
 __global__ __launch_bounds__(Cfg<N>::NT, Cfg<N>::MINB) void k_plain(PosArgs A, int k) {
     constexpr int NT = Cfg<N>::NT;
-    MSET_LDS(N, NT)
+    MSET_LDS(N, NT, 0)
     process_position<N, NT, K_PLAIN>(A, k, 0, 0, true, E, phx, phy, red);
 }
 
@@ -710,10 +723,11 @@ __global__ __launch_bounds__(Cfg<N>::NT, Cfg<N>::MINB) void k_plain(PosArgs A, i
 // nslab positions or a grid that divides nslab, so every slab's positions run on one workgroup in
 // increasing p: each slab sums the same positions in the same order whatever the grid size.
 // SEP: the separable-propagator routine (process_sep; the 256 x 256 engine, stored waves), its 1D factors in LDS.
-template <int N, int WV, bool SEP>
-__global__ __launch_bounds__(Cfg<N>::NT, Cfg<N>::MINB) void k_batch(PosArgs A) {
-    constexpr int NT = Cfg<N>::NT;
-    MSET_LDS(N, NT)
+// SH: the workgroup shape (Cfg).
+template <int N, int SH, int WV, bool SEP>
+__global__ __launch_bounds__((Cfg<N, SH>::NT), (Cfg<N, SH>::MINB)) void k_batch(PosArgs A) {
+    constexpr int NT = Cfg<N, SH>::NT;
+    MSET_LDS(N, NT, SH)
     if constexpr (SEP) {
         __shared__ float2 px[N], py[N];
         for (int i = threadIdx.x; i < 2 * N; i += NT) (i < N ? px : py)[i % N] = A.p1d[i];
@@ -738,7 +752,7 @@ __global__ __launch_bounds__(Cfg<N>::NT, Cfg<N>::MINB) void k_batch(PosArgs A) {
 template <int N, bool MT, bool SEP>
 __global__ __launch_bounds__(Cfg<N>::NT, Cfg<N>::MINB) void k_fwd(PosArgs A) {
     constexpr int NT = Cfg<N>::NT;
-    MSET_LDS(N, NT)
+    MSET_LDS(N, NT, 0)
     float2* px = nullptr;
     float2* py = nullptr;
     if constexpr (SEP) {
@@ -844,16 +858,20 @@ hipError_t launch_plain(int n, const PosArgs& A, int k, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_batch(int n, const PosArgs& A, int grid, hipStream_t st) {
+hipError_t launch_batch(int n, const PosArgs& A, int grid, int shape, hipStream_t st) {
+    if (shape != 0 && (shape != 1 || A.waves != WAVES_STORE)) return hipErrorInvalidValue;
     if (A.sep) {
         if (n != 256 || A.waves != WAVES_STORE || A.grad_mode != GRAD_REF || A.sset) return hipErrorInvalidValue;
-        hipLaunchKernelGGL((k_batch<256, WAVES_STORE, true>), dim3(grid), dim3(Cfg<256>::NT), 0, st, A);
+        if (shape) hipLaunchKernelGGL((k_batch<256, 1, WAVES_STORE, true>), dim3(grid), dim3(Cfg<256, 1>::NT), 0, st, A);
+        else hipLaunchKernelGGL((k_batch<256, 0, WAVES_STORE, true>), dim3(grid), dim3(Cfg<256>::NT), 0, st, A);
     } else if (A.waves == WAVES_RECOMPUTE) {
-        MSET_DISPATCH(n, hipLaunchKernelGGL((k_batch<NN, WAVES_RECOMPUTE, false>), dim3(grid), dim3(Cfg<NN>::NT), 0, st, A))
+        MSET_DISPATCH(n, hipLaunchKernelGGL((k_batch<NN, 0, WAVES_RECOMPUTE, false>), dim3(grid), dim3(Cfg<NN>::NT), 0, st, A))
     } else if (A.waves == WAVES_CHECKPOINT) {
         if (A.sset || A.ckseg < 2) return hipErrorInvalidValue;
-        MSET_DISPATCH(n, hipLaunchKernelGGL((k_batch<NN, WAVES_CHECKPOINT, false>), dim3(grid), dim3(Cfg<NN>::NT), 0, st, A))
-    } else MSET_DISPATCH(n, hipLaunchKernelGGL((k_batch<NN, WAVES_STORE, false>), dim3(grid), dim3(Cfg<NN>::NT), 0, st, A));
+        MSET_DISPATCH(n, hipLaunchKernelGGL((k_batch<NN, 0, WAVES_CHECKPOINT, false>), dim3(grid), dim3(Cfg<NN>::NT), 0, st, A))
+    } else if (shape) {
+        MSET_DISPATCH(n, hipLaunchKernelGGL((k_batch<NN, 1, WAVES_STORE, false>), dim3(grid), dim3(Cfg<NN, 1>::NT), 0, st, A))
+    } else MSET_DISPATCH(n, hipLaunchKernelGGL((k_batch<NN, 0, WAVES_STORE, false>), dim3(grid), dim3(Cfg<NN>::NT), 0, st, A));
     return hipGetLastError();
 }
 
@@ -880,8 +898,10 @@ hipError_t launch_plane_fft(int n, const float2* in, float2* out, const float2* 
     return hipGetLastError();
 }
 
-hipError_t occupancy_batch(int n, int* blocks_per_cu) {
-    MSET_DISPATCH(n, return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, (k_batch<NN, WAVES_STORE, false>), Cfg<NN>::NT, 0));
+hipError_t occupancy_batch(int n, int shape, int* blocks_per_cu) {
+    if (shape)
+        MSET_DISPATCH(n, return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, (k_batch<NN, 1, WAVES_STORE, false>), Cfg<NN, 1>::NT, 0));
+    MSET_DISPATCH(n, return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, (k_batch<NN, 0, WAVES_STORE, false>), Cfg<NN>::NT, 0));
     return hipSuccess;
 }
 

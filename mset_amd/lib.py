@@ -31,7 +31,7 @@ EXPORTS = ['mset_create', 'mset_destroy', 'mset_last_error', 'mset_stream', 'mse
            'mset_recon_set_engine']
 PATH_PLAIN, PATH_BATCHED = 0, 1
 EXIT_WAVES_AUTO, EXIT_WAVES_STORE, EXIT_WAVES_RECOMPUTE, EXIT_WAVES_CHECKPOINT = 0, 1, 2, 3
-ENGINE_AUTO, ENGINE_GENERAL = 0, 1
+ENGINE_AUTO, ENGINE_GENERAL, ENGINE_ALT_SHAPE = 0, 1, 2
 
 
 class MsetError(RuntimeError):

@@ -51,7 +51,7 @@ def test_tuner_times_every_candidate_restores_and_caches(tmp_path):
     cost = {(24, 8): 0.03, (24, 24): 0.02, (12, 8): 0.04, (12, 24): 0.005}
     r = FakeRecon(cost)
     path = str(tmp_path / 'tune.json')
-    best = autotune.tune_recon(r, batches=(8, 24), slots=[24, 12], cache_path=path)
+    best = autotune.tune_recon(r, batches=(8, 24), slots=[24, 12], engines=(0,), cache_path=path)
     assert (best['slots'], best['batch']) == (12, 24)
     assert r.slots == 12 and not r.dirty
     timed = [c for c in r.calls if c[0] == 'iterate']
@@ -66,7 +66,7 @@ def test_tuner_times_every_candidate_restores_and_caches(tmp_path):
     assert r.calls.count(('read', 'rotate')) == r.calls.count(('read', 'position_batch')) == 4
     # in-memory cache: no new passes; the disk cache holds the same choice
     n = len(r.calls)
-    assert autotune.tune_recon(r, batches=(8, 24), slots=[24, 12]) is best
+    assert autotune.tune_recon(r, batches=(8, 24), slots=[24, 12], engines=(0,)) is best
     assert not any(c[0] == 'iterate' for c in r.calls[n:])
     with open(path) as fh:
         disk = json.load(fh)
@@ -74,10 +74,23 @@ def test_tuner_times_every_candidate_restores_and_caches(tmp_path):
 
 
 def test_tuner_sweeps_the_plane_routine_at_256():
-    """At 256 x 256 both plane routines (separable and general) are candidates; the faster one is left set."""
+    """At 256 x 256 both plane routines (separable and general) and both workgroup shapes (32- and 16-line
+    tiles) are candidates; the fastest is left set."""
     from mset_amd import lib
-    cost = {(lib.ENGINE_AUTO, 24, 8): 0.03, (lib.ENGINE_GENERAL, 24, 8): 0.01}
+    cost = {(lib.ENGINE_AUTO, 24, 8): 0.03, (lib.ENGINE_ALT_SHAPE, 24, 8): 0.02, (lib.ENGINE_GENERAL, 24, 8): 0.01}
     r = FakeRecon(cost, shape=(16, 256, 256))
     best = autotune.tune_recon(r, batches=(8,), slots=[24])
-    assert len(best['table']) == 2 and best['engine'] == lib.ENGINE_GENERAL and r.engine == lib.ENGINE_GENERAL
-    assert autotune.engine_candidates(128) == (lib.ENGINE_AUTO,)
+    assert len(best['table']) == 3 and best['engine'] == lib.ENGINE_GENERAL and r.engine == lib.ENGINE_GENERAL
+    assert autotune.engine_name(lib.ENGINE_GENERAL | lib.ENGINE_ALT_SHAPE) == 'general routine, alternative shape'
+
+
+def test_tuner_sweeps_the_workgroup_shape_at_every_size():
+    """Below 256 the candidates are the two workgroup shapes of the plane size; the grid candidates start with
+    0, every resident workgroup of the engine's own shape."""
+    from mset_amd import lib
+    for n in (16, 24, 32, 48, 64, 96, 128):
+        assert autotune.engine_candidates(n) == (lib.ENGINE_AUTO, lib.ENGINE_ALT_SHAPE)
+    cost = {(lib.ENGINE_AUTO, 0, 8): 0.02, (lib.ENGINE_ALT_SHAPE, 0, 8): 0.005}
+    r = FakeRecon(cost, shape=(16, 128, 128))
+    best = autotune.tune_recon(r, batches=(8,), slots=[0])
+    assert best['engine'] == lib.ENGINE_ALT_SHAPE and r.engine == lib.ENGINE_ALT_SHAPE and r.slots == 0
