@@ -91,8 +91,8 @@ def main():
     ap.add_argument('--workload', default='C3', choices=['C2', 'C3'],
                     help='C3 (default): 256^2 probe, 21 tilts with uneven scans; C2: 128^2 probe, 11 tilts')
     ap.add_argument('--batch', type=int, default=None,
-                    help='positions per mini-batch (default: picked by the autotuner from 512..1024 for C3, '
-                         '512..1024 for C2)')
+                    help='positions per mini-batch (default: picked by the autotuner from 256 and 512 for C3, '
+                         'the range of batch sizes that workload names, and from 512 and 1024 for C2)')
     ap.add_argument('--no-autotune', action='store_true',
                     help='skip the autotuner: batch 512 (C3) / 1024 (C2) on every resident workgroup')
     ap.add_argument('--no-graph', action='store_true')
@@ -114,7 +114,7 @@ def main():
 
     from mset_amd import lib, physics, synth
     big = args.workload == 'C3'
-    batches = (512, 1024)
+    batches = (256, 512) if big else (512, 1024)
     if args.batch is not None:
         batches = (args.batch,)
     elif args.no_autotune:
@@ -195,11 +195,16 @@ def main():
     avg_ms = ms_b / nb
     b_alg = 32.0 * wl.P * wl.P            # bytes per probe-position-slice (store-exit-wave formulation)
     achieved = b_alg * per_launch_units / (avg_ms * 1e-3) / 1e9
+    # HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE summary of this kernel and workload:
+    # its measured bytes per probe-position-slice times this run's average units per launch (the profiled
+    # launches have the same batch size when the summary's command is the bench's own choice)
     traffic = None
     tpath = os.path.join(ROOT, 'profiles', 'traffic_position_batch_%s.json' % args.workload.lower())
     if os.path.exists(tpath):
         with open(tpath) as fh:
-            traffic = json.load(fh).get('bytes_per_launch')
+            tj = json.load(fh)
+        if tj.get('bytes_per_unit'):
+            traffic = tj['bytes_per_unit'] * per_launch_units
 
     plain = None
     if args.plain_tilts > 0:
