@@ -156,8 +156,9 @@ int mset_set_slot_limit(mset_ctx* ctx, int slots);
  * waves run the separable routine, which applies the two 1D half-steps of neighbouring planes in one pass
  * over the plane (half the passes).  GENERAL: always the 2D routine.  ALT_SHAPE: the batched kernel with
  * stored waves runs the alternative workgroup shape of its plane size: twice the threads per position of
- * the LDS engine up to 64, 1024 threads at 96, the LDS engine with 1024 threads instead of the register engine at
- * 128, 16-line tiles with 256 threads (three workgroups per CU instead of two) at 256.  The flags combine. */
+ * the LDS engine up to 64, 1024 threads at 96, at 128 the register engine with a half-size transpose
+ * buffer and a 128-register budget (two workgroups per CU instead of one), at 256 16-line tiles with 256
+ * threads (three workgroups per CU instead of two).  The flags combine. */
 #define MSET_ENGINE_AUTO 0
 #define MSET_ENGINE_GENERAL 1
 #define MSET_ENGINE_ALT_SHAPE 2

@@ -13,7 +13,7 @@ Tunables (both leave the kernels' arithmetic unchanged):
   engine mset_set_engine flags: the plane routine at 256 x 256 (the separable-propagator routine, one pass
          over the plane per propagation step, or the general 2D routine, two passes per step) and the
          workgroup shape of the batched kernel at every size (the default shape, or the alternative one:
-         threads per position of the LDS engine, the LDS engine instead of the register engine at 128,
+         threads per position of the LDS engine, two register-engine workgroups per CU at 128,
          16- instead of 32-line tiles at 256).  Engines sum in different orders: results agree to FP32
          rounding.
 

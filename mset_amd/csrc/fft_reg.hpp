@@ -61,11 +61,17 @@ __device__ __forceinline__ void transpose4(float2* v) {
     swap16(v[2].y, v[3].y);
 }
 
-template <int N> struct RegEngine;
+template <int N, bool HALF = false> struct RegEngine;
 
-template <> struct RegEngine<128> {
+// HALF: the transposes go through a buffer of half the size, real parts then imaginary parts (66 KB
+// instead of 132 KB), so that two workgroups fit in a CU's LDS
+template <bool HALF> struct RegEngine<128, HALF> {
     static constexpr int N = 128, E = 32, NT = 512, SP = 129;
-    static constexpr int LDS_ELEMS = N * SP;
+    static constexpr int LDS_ELEMS = HALF ? (N * SP + 1) / 2 : N * SP;
+    // the half-LDS engine runs two workgroups per CU at 128 registers: staged functors load their operands
+    // at most GMAX elements ahead and nothing ahead of the transform
+    static constexpr int GMAX = HALF ? 4 : E;
+    static constexpr bool AHEAD = !HALF;
     static constexpr bool FUSED = false;   // no fused passes: the plane is in registers
     float2* s;
     const float2* tw;  // W_128^k table in LDS
@@ -133,21 +139,54 @@ template <> struct RegEngine<128> {
 
     // C orientation, layout X (y = xidx, x = line) -> R orientation, layout D (y = line, x = h + 4 m)
     __device__ __forceinline__ void transpose_c_to_r() {
-        __syncthreads();
+        if constexpr (HALF) {
+            // a component's old values are dead once written, so the new ones take their registers
+            float* f = reinterpret_cast<float*>(s);
+            __syncthreads();
 #pragma unroll
-        for (int i = 0; i < E; ++i) s[A(xidx(i), line)] = v[i];
-        __syncthreads();
+            for (int i = 0; i < E; ++i) f[A(xidx(i), line)] = v[i].x;
+            __syncthreads();
 #pragma unroll
-        for (int m = 0; m < E; ++m) v[m] = s[A(line, h + 4 * m)];
+            for (int m = 0; m < E; ++m) v[m].x = f[A(line, h + 4 * m)];
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < E; ++i) f[A(xidx(i), line)] = v[i].y;
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < E; ++m) v[m].y = f[A(line, h + 4 * m)];
+        } else {
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < E; ++i) s[A(xidx(i), line)] = v[i];
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < E; ++m) v[m] = s[A(line, h + 4 * m)];
+        }
     }
     // R orientation, layout D (y = line, x = h + 4 m) -> C orientation, layout X (y = xidx, x = line)
     __device__ __forceinline__ void transpose_r_to_c() {
-        __syncthreads();
+        if constexpr (HALF) {
+            float* f = reinterpret_cast<float*>(s);
+            __syncthreads();
 #pragma unroll
-        for (int m = 0; m < E; ++m) s[A(line, h + 4 * m)] = v[m];
-        __syncthreads();
+            for (int m = 0; m < E; ++m) f[A(line, h + 4 * m)] = v[m].x;
+            __syncthreads();
 #pragma unroll
-        for (int i = 0; i < E; ++i) v[i] = s[A(xidx(i), line)];
+            for (int i = 0; i < E; ++i) v[i].x = f[A(xidx(i), line)];
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < E; ++m) f[A(line, h + 4 * m)] = v[m].y;
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < E; ++i) v[i].y = f[A(xidx(i), line)];
+        } else {
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < E; ++m) s[A(line, h + 4 * m)] = v[m];
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < E; ++i) v[i] = s[A(xidx(i), line)];
+        }
     }
 
     // Visit the 32 owned elements.  COL: C orientation, layout D, element i is (h + 4 i, line), index
@@ -192,10 +231,10 @@ template <> struct RegEngine<128> {
     // a staged functor over the 32 elements; the operands of the first PF are already in `ah`
     template <bool COL, bool MAP, bool LOADG, int PF, class F, class AH>
     __device__ __forceinline__ void run_staged(const F& f, const AH& ah) {
-        constexpr int G = F::G;
+        constexpr int G = F::G < GMAX ? F::G : GMAX;
         static_assert(E % G == 0 && PF % G == 0 && PF <= E, "group size must divide 32 and the prefetch");
         const unsigned vb = vbase();
-        if constexpr (F::EAGER && PF < E) {
+        if constexpr (F::EAGER && AHEAD && PF < E) {
             decltype(f.load(Pt{})) rest[E - PF];
 #pragma unroll
             for (int i = PF; i < E; ++i) rest[i - PF] = f.load(point<COL>(i, vb));
@@ -236,7 +275,7 @@ template <> struct RegEngine<128> {
     }
     // the post functor of a transform: its first PF operands are loaded before the last pass
     template <bool COL, class F, class PASS> __device__ __forceinline__ void last_pass_and_post(const F& f, PASS pass) {
-        if constexpr (IsStaged<F>::value && PrefetchOf<F>::value > 0) {
+        if constexpr (IsStaged<F>::value && PrefetchOf<F>::value > 0 && AHEAD) {
             Ahead<F> ah;
             load_ahead<COL>(f, ah);
             pass();

@@ -38,8 +38,9 @@ namespace mset {
 // Engine and workgroup shape per plane size.  NT threads; MINB: waves per SIMD the register budget must allow
 // (HIP launch bounds).  SH = 0: the shape every kernel uses; SH = 1: the alternative shape of the batched
 // kernel, a candidate for the autotuner (MSET_ENGINE_ALT_SHAPE): twice the threads per position of the LDS
-// engine up to N = 64, 1024 threads at 96, the LDS engine with 1024 threads instead of the register engine at 128,
-// and tiles of 16 lines with 256 threads (three workgroups per CU instead of two) at 256.
+// engine up to N = 64, 1024 threads at 96, at 128 the register engine with a half-size transpose buffer and a
+// 128-register budget (two workgroups per CU instead of one; it spills about 400 registers), and tiles of 16
+// lines with 256 threads (three workgroups per CU instead of two) at 256.
 template <int N, int SH = 0> struct Cfg;
 template <> struct Cfg<16> { static constexpr int NT = 64, MINB = 1;  using Eng = LdsEngine<16, 64>; };
 template <> struct Cfg<24> { static constexpr int NT = 64, MINB = 1;  using Eng = LdsEngine<24, 64>; };
@@ -55,7 +56,7 @@ template <> struct Cfg<32, 1> { static constexpr int NT = 128, MINB = 1; using E
 template <> struct Cfg<48, 1> { static constexpr int NT = 384, MINB = 1; using Eng = LdsEngine<48, 384>; };
 template <> struct Cfg<64, 1> { static constexpr int NT = 512, MINB = 1; using Eng = LdsEngine<64, 512>; };
 template <> struct Cfg<96, 1> { static constexpr int NT = 1024, MINB = 1; using Eng = LdsEngine<96, 1024>; };
-template <> struct Cfg<128, 1> { static constexpr int NT = 1024, MINB = 1; using Eng = LdsEngine<128, 1024>; };
+template <> struct Cfg<128, 1> { static constexpr int NT = 512, MINB = 4; using Eng = RegEngine<128, true>; };
 template <> struct Cfg<256, 1> { static constexpr int NT = 256, MINB = 3; using Eng = TiledEngine<256, 256>; };
 
 
