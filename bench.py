@@ -146,11 +146,13 @@ def main():
     use_graph = not args.no_graph
     r = make_recon(range(ntilt), STEPS_ALL)
     tuned = None
+    engine = lib.ENGINE_AUTO
     if not args.no_autotune:
         from mset_amd import autotune
         full = lib.batch_slots(wl.P)
         tuned = autotune.tune_recon(r, batches=batches, slots=[0, full // 2], log=log)
         args.batch = tuned['batch']
+        engine = tuned['engine']
 
         def slots_name(sl):
             return 'all resident' if sl == 0 else str(sl)
@@ -195,6 +197,16 @@ def main():
     avg_ms = ms_b / nb
     b_alg = 32.0 * wl.P * wl.P            # bytes per probe-position-slice (store-exit-wave formulation)
     achieved = b_alg * per_launch_units / (avg_ms * 1e-3) / 1e9
+    # A 256 x 256 wave does not fit on chip: every pass of the plane engine over it reads and writes the plane,
+    # 16 P^2 bytes.  Passes per position: 2L + 4 with the separable-propagator routine (one per propagation
+    # step each way), 4L + 3 with the general 2D routine.
+    off_chip = None
+    if wl.P == 256:
+        passes = (4 * wl.L + 3) if (engine & lib.ENGINE_GENERAL) else (2 * wl.L + 4)
+        b_off = b_alg + 16.0 * wl.P * wl.P * passes / wl.L
+        a_off = b_off * per_launch_units / (avg_ms * 1e-3) / 1e9
+        off_chip = dict(plane_passes_per_unit=passes / wl.L, bytes_per_unit=b_off, achieved=a_off,
+                        frac=a_off / HBM_PEAK_GBS)
     # HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE summary of this kernel and workload:
     # its measured bytes per probe-position-slice times this run's average units per launch (the profiled
     # launches have the same batch size when the summary's command is the bench's own choice)
@@ -241,7 +253,8 @@ def main():
                     autotune=tuned, error_after=err_after, rotate_ms_per_step=ms_r),
         roofline=dict(bound='hbm', achieved=achieved, peak=HBM_PEAK_GBS, unit='GB/s', frac=achieved / HBM_PEAK_GBS,
                       traffic=traffic, kernel='position_batch (k_batch<%d>)' % wl.P, kernel_avg_ms=avg_ms,
-                      units_per_launch=per_launch_units, bytes_per_unit=b_alg),
+                      units_per_launch=per_launch_units, bytes_per_unit=b_alg,
+                      with_plane_passes=off_chip),
         cpu_baseline=cpu, plain=plain, object_only=object_only,
     )
     print(json.dumps(out))
