@@ -279,7 +279,9 @@ PosArgs base_args(mset_ctx* c) {
 bool sep_fwd(const mset_ctx* c) {
     return plane_in_global(c->n) && c->prop_sep && !sset(c) && !(c->ws->engine & MSET_ENGINE_GENERAL);
 }
-bool sep_batch(const mset_ctx* c) { return sep_fwd(c) && c->p.grad_mode == GRAD_REF && c->ws->waves == WAVES_STORE; }
+bool sep_batch(const mset_ctx* c) {
+    return sep_fwd(c) && c->p.grad_mode == GRAD_REF && (c->ws->waves == WAVES_STORE || c->ws->waves == WAVES_CHECKPOINT);
+}
 
 // workgroups of the batched kernels the device keeps resident at once (1 per CU at N = 128)
 // workgroup shape of a batched call: the alternative shape (MSET_ENGINE_ALT_SHAPE) exists for stored waves

@@ -90,6 +90,7 @@ template <int N, int NT_> struct TiledEngine {
     const float2* tw;
     float2* RS;   // [y][x]
     float2* XS;   // [x][y]
+    float2* PB;   // the workgroup's plane buffers (bind)
 
     __device__ __forceinline__ void init(float2* lds, const float2* lds_tw) {
         s = lds;
@@ -97,8 +98,22 @@ template <int N, int NT_> struct TiledEngine {
     }
     // the workgroup's two plane buffers (2 N^2 elements)
     __device__ __forceinline__ void bind(float2* planes) {
+        PB = planes;
         RS = planes;
         XS = planes + N * N;
+    }
+    // the next chain pass of orientation COL reads its tile from `in` and writes it to `out` (each in the
+    // layout the plane buffer it replaces has; null: that plane buffer); unbind() goes back to the buffers
+    template <bool COL> __device__ __forceinline__ void redirect(const float2* in, float2* out) {
+        unbind();
+        float2*& src = COL ? RS : XS;
+        float2*& dst = COL ? XS : RS;
+        if (in) src = const_cast<float2*>(in);
+        if (out) dst = out;
+    }
+    __device__ __forceinline__ void unbind() {
+        RS = PB;
+        XS = PB + N * N;
     }
 
     // 256-point transform of the thread's elements j + 16 r (r < 16) of line `line`, in and out; LEAD: the

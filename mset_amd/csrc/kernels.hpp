@@ -67,14 +67,16 @@ struct PosArgs {
 enum { WAVES_STORE = 0, WAVES_RECOMPUTE = 1, WAVES_CHECKPOINT = 2 };
 // per-slot scratch planes of the recompute variant: psi0, Fourier psi0, two wave planes, parked adjoint
 constexpr int RECOMPUTE_PLANES = 5;
-// checkpoint variant: segment length K = ceil(sqrt(L + 1)) and its per-slot planes: psi0, Fourier psi0, the
-// checkpoints psi_K, psi_2K, .. psi_(L / K) K, the K - 1 other waves of one segment, the parked adjoint wave
+// checkpoint variant: segment length K = ceil(sqrt(L + 1)) and its per-slot planes.  General routine: psi0,
+// Fourier psi0, the checkpoints psi_K, psi_2K, .. psi_(L / K) K, the K - 1 other waves of one segment, the
+// parked adjoint wave (the last plane).  Separable routine: psi0, the checkpoint pass inputs of the segments
+// below the last ((L - 1) / K), the K waves of one segment, two planes for the rebuild passes.
 __host__ __device__ inline int checkpoint_segment(int L) {
     int k = 2;
     while (k * k < L + 1) ++k;
     return k;
 }
-__host__ __device__ inline int checkpoint_planes(int L, int K) { return L / K + K + 2; }
+__host__ __device__ inline int checkpoint_planes(int L, int K) { return L / K + K + 3; }
 
 bool plane_size_supported(int n);
 int position_threads(int n);

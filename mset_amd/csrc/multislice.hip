@@ -508,9 +508,17 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
 // Arrays kept in the other layout than the pass that reads them (fftn(probe) and sqrt(M), [x][y], in a
 // column pass; psi0, [y][x], written in a row pass) go through the LDS tile (TileLoad / TileStore).
 //
-// Used for the batched gradient with the reference pairing and stored waves, and for the forward-only
-// kernels; the sequential kernel, the exact pairing and the recomputed waves use process_position.
-template <int N, int NT, int KIND, class Eng>
+// Checkpointed waves (WV = WAVES_CHECKPOINT): segments of K waves, s = (i - 1) / K for psi_i, i = 1 .. L.
+// Pass X_(i+1) makes psi_i from its input I_(i+1); the forward sweep writes the input I_(sK+2) of each
+// segment below the last straight into a checkpoint plane (the pass before it stores there instead of in
+// the plane buffer) and keeps the last segment's waves.  On entering a lower segment the adjoint reruns its
+// K forward passes from the checkpoint, their outputs alternating between two rebuild planes, each storing
+// its wave: the forward sweep's own arithmetic, about one forward sweep more in all.
+//
+// Used for the batched gradient with the reference pairing and stored or checkpointed waves, and for the
+// forward-only kernels; the sequential kernel, the exact pairing and the recomputed waves use
+// process_position.
+template <int N, int NT, int KIND, int WV = WAVES_STORE, class Eng>
 __device__ __forceinline__ void process_sep(const PosArgs& A, int k, int slot, int slab, bool first, Eng& E,
                                             float2* phx, float2* phy, const float2* px, const float2* py,
                                             double* red) {
@@ -523,7 +531,26 @@ __device__ __forceinline__ void process_sep(const PosArgs& A, int k, int slot, i
     const int L = A.L;
     const float inv_n = 1.0f / (float)N, inv_n2 = 1.0f / (float)N2;
     constexpr bool GRAD = KIND == K_BATCH;
-    float2* save = A.save + (size_t)slot * (L + 2) * N2;
+    constexpr bool ckpt = GRAD && WV == WAVES_CHECKPOINT;
+    static_assert(WV == WAVES_STORE || WV == WAVES_CHECKPOINT, "separable path: stored or checkpointed waves");
+    const int K = ckpt ? A.ckseg : 1, SL = ckpt ? (L - 1) / K : 0;
+    // Per-slot scratch.  Store: plane i holds psi_i, i = 0 .. L.  Checkpoint: plane 0 psi0, planes 1 + s the
+    // checkpoint inputs I_(sK+2) (s < SL), planes 1 + SL + (i - 1) % K the waves psi_i of the segment held,
+    // planes 1 + SL + K and 2 + SL + K the rebuild passes' outputs.
+    float2* save = A.save + (size_t)slot * (ckpt ? checkpoint_planes(L, K) : L + 2) * N2;
+    auto wave = [&](int i) -> float2* {
+        if constexpr (ckpt) return save + (size_t)(i > 0 ? 1 + SL + (i - 1) % K : 0) * N2;
+        else return save + (size_t)i * N2;
+    };
+    // checkpoint: whether pass X_k's input I_k is a checkpoint, and where pass X_k's output I_(k+1) goes
+    auto ck_in = [&](int kk) -> const float2* {
+        return ckpt && kk >= 2 && (kk - 2) % K == 0 && (kk - 2) / K < SL ? save + (size_t)(1 + (kk - 2) / K) * N2 : nullptr;
+    };
+    auto ck_out = [&](int kk) -> float2* {
+        return ckpt && (kk - 1) % K == 0 && (kk - 1) / K < SL ? save + (size_t)(1 + (kk - 1) / K) * N2 : nullptr;
+    };
+    // psi_i is kept by the forward sweep (checkpoint: only the last segment's)
+    auto keeps = [&](int i) { return GRAD && (!ckpt || (i - 1) / K == SL); };
     E.bind(A.planes + (size_t)slot * 2 * N2);
     const bool want_pos = GRAD && (A.want & WANT_POS) != 0;
     const bool want_probe = GRAD && (A.want & WANT_PROBE) != 0;
@@ -570,6 +597,7 @@ __device__ __forceinline__ void process_sep(const PosArgs& A, int k, int slot, i
         const BufF t0(A.tph, N2);
         auto scale = [&](const Pt&, float2 v) { return cscale(v, inv_n2); };
         auto times_t = staged<16>([&](const Pt& p) { return t0[p.e]; }, [&](const Pt&, float2 v, float o) { return cmul(v, expi(o)); });
+        if constexpr (ckpt) E.template redirect<false>(nullptr, ck_out(1));
         E.template chain<false, SWAP, INV, FWD, INV>(NoOp(), seq(scale, TileStore{GRAD ? save : nullptr}, times_t),
                                                      pfac(RT{}));
     }
@@ -584,22 +612,72 @@ __device__ __forceinline__ void process_sep(const PosArgs& A, int k, int slot, i
         });
         E.template chain<COL, SWAP, FWD, INV, FWD, INV>(NoOp(), pfac(colt), mid, pfac(colt));
     };
-    for (int a = 1; a < L; ++a) {
-        if ((a + 1) % 2 == 0) fwd_pass(CT{}, a);
-        else fwd_pass(RT{}, a);
+    // the same pass with its input and output redirected and its wave kept or not (checkpoint variant)
+    auto fwd_pass_io = [&](auto colt, int a, bool keep, const float2* in, float2* out) {
+        constexpr bool COL = decltype(colt)::value;
+        const BufF ta(A.tph + (size_t)a * N2, N2);
+        const BufC sv(wave(a), N2);
+        auto mid = staged<16>([&](const Pt& p) { return ta[p.e]; }, [&](const Pt& p, float2 v, float o) {
+            if (keep) sv.put(p.e, v);
+            return cmul(v, expi(o));
+        });
+        E.template redirect<COL>(in, out);
+        E.template chain<COL, SWAP, FWD, INV, FWD, INV>(NoOp(), pfac(colt), mid, pfac(colt));
+    };
+    if constexpr (ckpt) {
+        for (int a = 1; a < L; ++a) {
+            if ((a + 1) % 2 == 0) fwd_pass_io(CT{}, a, keeps(a), ck_in(a + 1), ck_out(a + 1));
+            else fwd_pass_io(RT{}, a, keeps(a), ck_in(a + 1), ck_out(a + 1));
+        }
+    } else {
+        for (int a = 1; a < L; ++a) {
+            if ((a + 1) % 2 == 0) fwd_pass(CT{}, a);
+            else fwd_pass(RT{}, a);
+        }
     }
     // ---- X_(L+1): finish plane L-1's step, store psi_L, forward transform along this axis
     auto last_pass = [&](auto colt) {
         constexpr bool COL = decltype(colt)::value;
-        const BufC sv(save + (size_t)L * N2, N2);
+        const BufC sv(wave(L), N2);
         auto keep = [&](const Pt& p, float2 v) {
             if (GRAD) sv.put(p.e, v);
             return v;
         };
+        if constexpr (ckpt) E.template redirect<COL>(ck_in(L + 1), nullptr);
         E.template chain<COL, SWAP, FWD, INV, FWD>(NoOp(), pfac(colt), keep);
     };
     if ((L + 1) % 2 == 0) last_pass(CT{});
     else last_pass(RT{});
+    if constexpr (ckpt) E.unbind();
+
+    // checkpoint: rebuild segment s's waves psi_(sK+1) .. psi_(sK+K) by its forward passes X_(sK+2) ..
+    // X_(sK+K+1) from the checkpoint I_(sK+2); the segment's last pass stops after its wave
+    auto rebuild = [&](int s) {
+        const float2* src = save + (size_t)(1 + s) * N2;
+        for (int n = 0; n < K; ++n) {
+            const int kk = s * K + 2 + n, i = kk - 1;
+            if (n + 1 < K) {
+                float2* dst = save + (size_t)(1 + SL + K + (n & 1)) * N2;
+                if (kk % 2 == 0) fwd_pass_io(CT{}, i, true, src, dst);
+                else fwd_pass_io(RT{}, i, true, src, dst);
+                src = dst;
+            } else {
+                const BufC sv(wave(i), N2);
+                auto keep = [&](const Pt& p, float2 v) {
+                    sv.put(p.e, v);
+                    return v;
+                };
+                if (kk % 2 == 0) {
+                    E.template redirect<true>(src, nullptr);
+                    E.template chain<true, NOST, FWD, INV>(NoOp(), pfac(CT{}), keep);
+                } else {
+                    E.template redirect<false>(src, nullptr);
+                    E.template chain<false, NOST, FWD, INV>(NoOp(), pfac(RT{}), keep);
+                }
+            }
+        }
+        E.unbind();
+    };
 
     // ---- X_(L+2): forward along the other axis -> exit wave Psi; intensity, error, residual
     // (STEP02_FORWARD_4D.m:36-49); the residual times conj(p) and the inverse along this axis start the adjoint
@@ -637,7 +715,7 @@ __device__ __forceinline__ void process_sep(const PosArgs& A, int k, int slot, i
     auto adj_pass = [&](auto colt, int a) {
         constexpr bool COL = decltype(colt)::value;
         const BufF tb(A.tph_b + (size_t)a * N2, N2);
-        const BufC ws(save + (size_t)(a + 1) * N2, N2);              // reference pairing: the wave after plane a
+        const BufC ws(wave(a + 1), N2);                               // reference pairing: the wave after plane a
         const BufF g(A.gslab + ((size_t)slab * L + a) * N2, N2);
         const BufF gin(A.gslab + ((size_t)slab * L + a) * N2, first ? 0u : N2);   // empty view: reads 0
         auto mid = staged<8>([&](const Pt& p) { return AdjPh{tb[p.e], ws[p.e], gin[p.e]}; },
@@ -673,9 +751,21 @@ __device__ __forceinline__ void process_sep(const PosArgs& A, int k, int slot, i
             else E.template chain<false, SWAP, FWD, INV, FWD, INV>(NoOp(), bfac(colt), mid, bfac(colt));
         }
     };
-    for (int a = L - 1; a >= 0; --a) {
-        if (a % 2 == 0) adj_pass(CT{}, a);
-        else adj_pass(RT{}, a);
+    if constexpr (ckpt) {
+        int cseg = SL;     // the segment whose waves are held
+        for (int a = L - 1; a >= 0; --a) {
+            if (a / K != cseg) {
+                cseg = a / K;                                         // psi_(a+1): segment (a + 1 - 1) / K
+                rebuild(cseg);
+            }
+            if (a % 2 == 0) adj_pass(CT{}, a);
+            else adj_pass(RT{}, a);
+        }
+    } else {
+        for (int a = L - 1; a >= 0; --a) {
+            if (a % 2 == 0) adj_pass(CT{}, a);
+            else adj_pass(RT{}, a);
+        }
     }
     if (want_pos) {
         dr = block_sum<NT>(dr, red);
@@ -735,7 +825,7 @@ __global__ __launch_bounds__((Cfg<N, SH>::NT), (Cfg<N, SH>::MINB)) void k_batch(
         __syncthreads();
         for (int p = blockIdx.x; p < A.count; p += gridDim.x) {
             const int k = A.order ? A.order[A.first + p] : A.first + p;
-            process_sep<N, NT, K_BATCH>(A, k, blockIdx.x, p % A.nslab, p < A.nslab, E, phx, phy, px, py, red);
+            process_sep<N, NT, K_BATCH, WV>(A, k, blockIdx.x, p % A.nslab, p < A.nslab, E, phx, phy, px, py, red);
             __syncthreads();
         }
     } else {
@@ -862,9 +952,17 @@ hipError_t launch_plain(int n, const PosArgs& A, int k, hipStream_t st) {
 hipError_t launch_batch(int n, const PosArgs& A, int grid, int shape, hipStream_t st) {
     if (shape != 0 && (shape != 1 || A.waves != WAVES_STORE)) return hipErrorInvalidValue;
     if (A.sep) {
-        if (n != 256 || A.waves != WAVES_STORE || A.grad_mode != GRAD_REF || A.sset) return hipErrorInvalidValue;
-        if (shape) hipLaunchKernelGGL((k_batch<256, 1, WAVES_STORE, true>), dim3(grid), dim3(Cfg<256, 1>::NT), 0, st, A);
-        else hipLaunchKernelGGL((k_batch<256, 0, WAVES_STORE, true>), dim3(grid), dim3(Cfg<256>::NT), 0, st, A);
+        if (n != 256 || A.grad_mode != GRAD_REF || A.sset) return hipErrorInvalidValue;
+        if (A.waves == WAVES_CHECKPOINT) {
+            if (shape || A.ckseg < 2) return hipErrorInvalidValue;
+            hipLaunchKernelGGL((k_batch<256, 0, WAVES_CHECKPOINT, true>), dim3(grid), dim3(Cfg<256>::NT), 0, st, A);
+        } else if (A.waves != WAVES_STORE) {
+            return hipErrorInvalidValue;
+        } else if (shape) {
+            hipLaunchKernelGGL((k_batch<256, 1, WAVES_STORE, true>), dim3(grid), dim3(Cfg<256, 1>::NT), 0, st, A);
+        } else {
+            hipLaunchKernelGGL((k_batch<256, 0, WAVES_STORE, true>), dim3(grid), dim3(Cfg<256>::NT), 0, st, A);
+        }
     } else if (A.waves == WAVES_RECOMPUTE) {
         MSET_DISPATCH(n, hipLaunchKernelGGL((k_batch<NN, 0, WAVES_RECOMPUTE, false>), dim3(grid), dim3(Cfg<NN>::NT), 0, st, A))
     } else if (A.waves == WAVES_CHECKPOINT) {

@@ -242,22 +242,22 @@ def test_c4_gradients_vs_central_differences():
 def test_c4_store_vs_recompute_at_256_cubed():
     """The memory case of the adjoint-verification configuration: a 256^3 volume with one-voxel slices (256
     propagation planes), probe 256^2, 64 positions.  Recomputing the exit waves keeps 5 wave planes per
-    resident position instead of 258, checkpointing 34 (K = 17); both give the stored-wave gradient, and the
+    resident position instead of 258, checkpointing 35 (K = 17); both give the stored-wave gradient, and the
     three variants match the FP64 oracle on two positions."""
     from mset_amd import exitwaves, lib
     wl = exitwaves.make_workload(64)
     out = exitwaves.run(64, repeat=1, wl=wl)
     print('256^3 store vs recompute vs checkpoint:',
           {k: out[k] for k in ('wave_memory_ratio', 'scratch_memory_ratio', 'time_ratio', 'recompute_vs_store',
-                               'checkpoint_wave_memory_ratio', 'checkpoint_time_ratio', 'checkpoint_vs_store_general')},
+                               'checkpoint_wave_memory_ratio', 'checkpoint_time_ratio', 'checkpoint_vs_store')},
           flush=True)
     assert wl.L == 256 and out['wave_memory_ratio'] == (wl.L + 2) / 5
-    assert out['checkpoint_wave_memory_ratio'] == (wl.L + 2) / 34
+    assert out['checkpoint_wave_memory_ratio'] == (wl.L + 2) / 35
     assert out['variants']['store']['resolved'] == lib.EXIT_WAVES_STORE
     assert out['variants']['recompute']['resolved'] == lib.EXIT_WAVES_RECOMPUTE
     assert out['variants']['checkpoint']['resolved'] == lib.EXIT_WAVES_CHECKPOINT
-    # checkpointed waves are the general routine's stored waves, rebuilt by the same forward steps
-    ck = out['checkpoint_vs_store_general']
+    # checkpointed waves are the stored ones, rebuilt by the same forward passes (separable routine)
+    ck = out['checkpoint_vs_store']
     assert ck['dV_rel'] < 1e-5 and ck['dprobe_rel'] < 1e-5 and ck['loss_rel'] < 1e-6
     # the stored-wave variant runs the separable-propagator routine, the recomputing one the general routine:
     # their forward sweeps round differently (~1e-8 per plane over 256 planes), both far inside 1e-4 of FP64

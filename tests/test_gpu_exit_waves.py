@@ -33,7 +33,7 @@ def ck_planes(L):
     k = 2
     while k * k < L + 1:
         k += 1
-    return L // k + k + 2
+    return L // k + k + 3
 
 
 def _upload(ctx, c, mode, prop=None, steps=(0.0, 0.0, 0.0)):
@@ -82,34 +82,39 @@ def test_recomputed_waves_give_the_stored_gradient(n, nz, binning, mode):
                                                (64, 12, 1, O.MODE_EXACT), (32, 40, 1, O.MODE_EXACT),
                                                (48, 15, 1, O.MODE_REF), (128, 16, 2, O.MODE_EXACT),
                                                (128, 36, 2, O.MODE_REF), (256, 12, 2, O.MODE_REF),
-                                               (256, 10, 1, O.MODE_EXACT)])
+                                               (256, 10, 1, O.MODE_EXACT), (256, 13, 1, O.MODE_REF),
+                                               (256, 18, 2, O.MODE_REF)])
 def test_checkpointed_waves_give_the_stored_gradient(n, nz, binning, mode):
     """Segments that end on the last plane (L a multiple of K) and ones that do not, both pairings, every
-    engine; at 256 the stored variant is run on the general routine, which the checkpointed one uses."""
+    engine; at 256 with the reference pairing both plane routines (the separable one reruns its one-axis
+    passes from checkpointed pass inputs), each against the stored variant on the same routine."""
     from mset_amd import lib
     c = small_case(n=n, nz=nz, binning=binning, npos=6, seed=7 * n + nz, spread=n / 8)
     L = -(-nz // binning)
     print('case: checkpointed exit waves', n, nz, binning, mode, 'L', L, flush=True)
+    odV, odp, odpos, _ = O.grad_fixed(c['V'], c['probe'], c['prop'], c['bprop'], c['xl'], c['yl'], c['M'],
+                                      c['bin'], c['sig'], c['dx'], mode=mode)
+    engines = [lib.ENGINE_GENERAL] + ([lib.ENGINE_AUTO] if n == 256 and mode == O.MODE_REF else [])
     ctx = lib.Context(0)
     try:
         _upload(ctx, c, mode)
-        ctx.set_engine(lib.ENGINE_GENERAL)
-        ctx.set_exit_waves(lib.EXIT_WAVES_STORE)
-        dV, dp, dpos, err = ctx.grad()
-        ctx.set_exit_waves(lib.EXIT_WAVES_CHECKPOINT)
-        cV, cp, cpos, cerr = ctx.grad()
-        mc, bc = ctx.exit_waves()
+        for e in engines:
+            ctx.set_engine(e)
+            ctx.set_exit_waves(lib.EXIT_WAVES_STORE)
+            dV, dp, dpos, err = ctx.grad()
+            ctx.set_exit_waves(lib.EXIT_WAVES_CHECKPOINT)
+            cV, cp, cpos, cerr = ctx.grad()
+            mc, bc = ctx.exit_waves()
+            assert mc == lib.EXIT_WAVES_CHECKPOINT and bc == n * n * (ck_planes(L) * 8 + L * 4)
+            print('  engine %d, checkpointed vs stored: dV %.2e dprobe %.2e dpos %.2e loss %.2e bitwise dV %s'
+                  % (e, rel(cV, dV), rel(cp, dp), rel(cpos, dpos), abs(cerr / err - 1), np.array_equal(cV, dV)),
+                  flush=True)
+            assert abs(cerr / err - 1) < CKPT_TOL
+            assert rel(cV, dV) < CKPT_TOL and rel(cp, dp) < CKPT_TOL
+            assert np.max(np.abs(cpos - dpos)) <= CKPT_TOL * np.max(np.abs(dpos))
+            assert rel(cV, odV) < GRAD_TOL and rel(cp, odp) < GRAD_TOL and rel(cpos, odpos) < GRAD_TOL
     finally:
         ctx.close()
-    assert mc == lib.EXIT_WAVES_CHECKPOINT and bc == n * n * (ck_planes(L) * 8 + L * 4)
-    print('checkpointed vs stored: dV %.2e dprobe %.2e dpos %.2e loss %.2e bitwise dV %s'
-          % (rel(cV, dV), rel(cp, dp), rel(cpos, dpos), abs(cerr / err - 1), np.array_equal(cV, dV)), flush=True)
-    assert abs(cerr / err - 1) < CKPT_TOL
-    assert rel(cV, dV) < CKPT_TOL and rel(cp, dp) < CKPT_TOL
-    assert np.max(np.abs(cpos - dpos)) <= CKPT_TOL * np.max(np.abs(dpos))
-    odV, odp, odpos, _ = O.grad_fixed(c['V'], c['probe'], c['prop'], c['bprop'], c['xl'], c['yl'], c['M'],
-                                      c['bin'], c['sig'], c['dx'], mode=mode)
-    assert rel(cV, odV) < GRAD_TOL and rel(cp, odp) < GRAD_TOL and rel(cpos, odpos) < GRAD_TOL
 
 
 def test_checkpoints_serve_a_propagator_that_recompute_refuses():
