@@ -132,7 +132,7 @@ int mset_waves(mset_ctx* ctx, int k, float* waves_out);
  * inverting the forward sweep, psi_a = conj(t_a) .* ifftn(fftn(psi_{a+1}) .* conj(prop)), for 50 % more
  * transforms.  It needs a unit-modulus propagator (the Fresnel propagator of Func_generate_propagator.m
  * is one; EINVAL otherwise).  CHECKPOINT keeps every K-th wave (K = ceil(sqrt(L + 1))) and the last
- * segment of K, L / K + K + 2 planes, and rebuilds each earlier segment from its checkpoint with the
+ * segment of K, L / K + K + 3 planes, and rebuilds each earlier segment from its checkpoint with the
  * forward sweep's own steps: any propagator, the stored variant's waves, about one forward sweep more.
  * AUTO (the default) stores unless the store scratch (waves and gradient slabs) would exceed a quarter of
  * the device memory; then it checkpoints if that scratch fits, else recomputes (unit-modulus propagator),
