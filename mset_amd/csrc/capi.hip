@@ -283,13 +283,13 @@ bool sep_batch(const mset_ctx* c) {
     return sep_fwd(c) && c->p.grad_mode == GRAD_REF && (c->ws->waves == WAVES_STORE || c->ws->waves == WAVES_CHECKPOINT);
 }
 
-// workgroups of the batched kernels the device keeps resident at once (1 per CU at N = 128)
 // workgroup shape of a batched call: the alternative shape (MSET_ENGINE_ALT_SHAPE) exists for stored waves
 int shape_for(const mset_ctx* c, int waves) {
     return (c->ws->engine & MSET_ENGINE_ALT_SHAPE) && waves == WAVES_STORE ? 1 : 0;
 }
 
-// resident workgroups of the batched kernel of a workgroup shape (by default, the last batched call's)
+// workgroups of the batched kernel of a workgroup shape the device keeps resident at once (1 per CU at
+// N = 128 in the default shape); by default the shape of the last batched call
 int device_slots(mset_ctx* c, int shape) {
     int per_cu = 1;
     if (occupancy_batch(c->n, shape, &per_cu) != hipSuccess || per_cu < 1) per_cu = 1;
@@ -324,8 +324,8 @@ int launch_grid(int want, int count, int S) {
 
 int ensure_fixed_buffers(mset_ctx* c) {
     Workspace& W = *c->ws;
-    // the separable kernels keep the transmission in two layouts (forward and adjoint passes)
-    // complex planes, or (separable routine) f32 phases in two layouts: L planes either way
+    // complex planes, or (separable routine) f32 phases in the forward and the adjoint layouts: L planes
+    // either way
     CHK(W.trans.ensure(plane(c) * c->L, "transmission"));
     CHK(W.probeF.ensure(plane(c), "probe spectrum"));
     // the position kernels of the largest planes keep the plane in global memory: 2 planes per slot
