@@ -25,6 +25,15 @@
 // STEP03_BACK.m:14-15 pairs the back-wave of plane a with the wave *after* plane a, which is not
 // the gradient of its own loss) and the gradient-at-fixed-state sums used by the batched path.
 //
+// How each is pinned (tests/test_oracle.py): the propagator, probe, sub-pixel shift, transmission,
+// forward waves, losses and rotation against the reference's CuAu example vectors (tests/golden/);
+// the exact gradients against central finite differences; the reference-mode object term by the
+// pairing identity with the exact one; the reference-mode position term against numpy.gradient (the
+// MATLAB edge rule) on a wave with energy at the field border; oracle_run_tilt against a loop of
+// one-position gradients that advances the state after each position; the SSET pair by reduction to
+// the multislice one and finite differences; tv_method by the ROF closed form and its invariants; the
+// rotation also by 90-degree permutations and round trips.  No function is left unpinned.
+//
 // Every function is templated on the real type: double is the correctness oracle, float restates
 // the reference's single-precision arithmetic and is the timed CPU baseline.  FFTs are a
 // self-contained mixed-radix Stockham FFT with long-double twiddles (no FFTW here).
