@@ -1,6 +1,8 @@
 """GPU parity at the named configurations: C1 (the reference's CPU-runnable case, one update pass over all
-tilts), C3 (FP32 device gradient against the FP64 oracle at the 256^2, 32-plane shape) and C4 (device
-gradients against central finite differences of the FP64 loss along 50 seeded directions, h = 1e-4)."""
+tilts), C2 (a slot-reused tilt against the oracle; full-size repeatability, additivity and graph replay), C3
+(the full series' graph-captured pass against the eager one; FP32 device gradient against the FP64 oracle
+at the 256^2, 32-plane shape) and C4 (device gradients against central finite differences of the FP64 loss
+along 50 seeded directions, h = 1e-4; stored, recomputed and checkpointed waves at 256^3)."""
 import numpy as np
 import pytest
 
@@ -119,6 +121,45 @@ def test_c2_full_size_properties():
                 assert np.array_equal(gx, ex), rep
     finally:
         r.close()
+
+
+def test_c3_full_size_graph_replay_equals_eager():
+    """The headline configuration end to end: the full C3 series (21 tilts, 25,767 positions on uneven scans,
+    256^2 probes, 32 planes), batches of 512 on the separable routine with all three gradient terms.  A
+    graph-captured update pass replays to the eager pass's volume, probes and positions bit for bit, twice,
+    with the one-launch error pass (eager work) between the replays; the error after the pass is finite and
+    below the error before it."""
+    from mset_amd import lib
+    ctx = lib.Context(0)
+    wl = _workload('C3', ctx)
+    ctx.close()
+    V = np.zeros_like(wl.truth, dtype=np.float32)
+    steps = (100.0, 1e-3, 10.0)
+    r = lib.Recon(V, wl.prop, len(wl.angles), wl.sigma, wl.dx, slice_binning=wl.binning, step_size=steps)
+    try:
+        for i, ang in enumerate(wl.angles):
+            r.set_tilt(i, ang, wl.probe, wl.xlists[i], wl.ylists[i], wl.measured[i], n_scan=wl.n_scans[i])
+        before = float(np.mean(r.error_pass()))
+        r.snapshot()
+        print('case: C3 eager pass', flush=True)
+        r.iterate(lib.PATH_BATCHED, 512, use_graph=False, want_error=False)
+        eager = r.get()
+        after = float(np.mean(r.error_pass()))
+        r.restore()
+        r.iterate(lib.PATH_BATCHED, 512, use_graph=True, want_error=False)       # eager pass, then capture
+        for rep in range(2):
+            print('case: C3 graph replay', rep, flush=True)
+            r.restore()
+            r.error_pass()                                                       # eager work between replays
+            r.iterate(lib.PATH_BATCHED, 512, use_graph=True, want_error=False)   # replay
+            got = r.get()
+            assert np.array_equal(got[0], eager[0]) and np.array_equal(got[1], eager[1]), rep
+            for gx, ex in zip(got[2] + got[3], eager[2] + eager[3]):
+                assert np.array_equal(gx, ex), rep
+    finally:
+        r.close()
+    print('C3 error before %.5f after one pass %.5f' % (before, after), flush=True)
+    assert np.isfinite(after) and after < before
 
 
 def test_c3_fp32_gradient_vs_fp64_oracle():
