@@ -268,6 +268,7 @@ PosArgs base_args(mset_ctx* c) {
     A.planes = plane_in_global(c->n) ? c->ws->planes.p : nullptr;
     A.sep = 0;
     A.p1d = c->p1d.p;
+    A.pfac = c->prop_sep && !(c->ws->engine & MSET_ENGINE_GENERAL) ? 1 : 0;
     // the separable routine's phases share the transmission buffer: forward layouts, then adjoint layouts
     A.tph = (const float*)c->ws->trans.p;
     A.tph_b = A.tph + (size_t)c->L * plane(c);

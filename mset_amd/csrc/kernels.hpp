@@ -53,7 +53,8 @@ struct PosArgs {
     int nslab;                   // K_BATCH: position p of the launch adds to slab p % nslab
     int sep;                     // N = 256: separable-propagator routine (p1d, tph in the forward layouts,
                                  // tph_b in the adjoint layouts; odd gradient planes transposed)
-    const float2* p1d;           // sep: the propagator's 1D factors px[N], py[N] (p(kx, ky) = px(kx) py(ky))
+    const float2* p1d;           // sep / pfac: the propagator's 1D factors px[N], py[N] (p(kx, ky) = px(kx) py(ky))
+    int pfac;                    // N < 256, batched gradient with stored waves: apply the propagator as p1d's factors
     const float* tph;            // sep: transmission phases / pi, reduced to [-1, 1] (t = exp(i pi tph)), forward layouts
     const float* tph_b;          // sep, K_BATCH: the same phases in the adjoint passes' layouts
     float2* planes;              // N = 256: per-slot plane buffers (2 N^2), the tiled engine's state

@@ -96,10 +96,12 @@ struct AdjPh {
 
 // slot: the workgroup's scratch (waves, plane buffers); slab: the gradient slab the position adds to;
 // first: the slab's first position of the launch, which writes the slab instead of adding to it (so the
-// slabs need no clearing launch)
-template <int N, int NT, int KIND, int WV = WAVES_STORE, class Eng>
+// slabs need no clearing launch).  PFAC: the propagator is applied as the product of its 1D factors
+// pxs[kx] pys[ky] (in LDS) instead of being read from the 2D array (batched gradient, separable propagator).
+template <int N, int NT, int KIND, int WV = WAVES_STORE, bool PFAC = false, class Eng>
 __device__ __forceinline__ void process_position(const PosArgs& A, int k, int slot, int slab, bool first, Eng& E,
-                                                 float2* phx, float2* phy, double* red) {
+                                                 float2* phx, float2* phy, double* red,
+                                                 const float2* pxs = nullptr, const float2* pys = nullptr) {
     constexpr int N2 = N * N;
     const int t = threadIdx.x;
     const int L = A.L;
@@ -199,10 +201,17 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
 
     // ---- forward sweep: psi_a = ifftn(fftn(psi_{a-1} .* t_a) .* p); the engine enters plane a holding
     // psi_{a-1} .* t_a
+    auto pfac = [&](const Pt& p) { return cmul(pxs[p.x], pys[p.y]); };
     for (int a = 0; a < (multi ? L : 0); ++a) {
-        auto byp = staged<16, 32>([&](const Pt& p) { return pTb[p.e]; }, [&](const Pt&, float2 v, float2 o) { return cmul(v, o); });
-        if constexpr (FUSE) E.row_step(byp);
-        else E.fwd(NoOp(), byp);
+        if constexpr (PFAC) {
+            auto byp = [&](const Pt& p, float2 v) { return cmul(v, pfac(p)); };
+            if constexpr (FUSE) E.row_step(byp);
+            else E.fwd(NoOp(), byp);
+        } else {
+            auto byp = staged<16, 32>([&](const Pt& p) { return pTb[p.e]; }, [&](const Pt&, float2 v, float2 o) { return cmul(v, o); });
+            if constexpr (FUSE) E.row_step(byp);
+            else E.fwd(NoOp(), byp);
+        }
         const bool next = a + 1 < L;
         // recompute keeps only the exit wave psi_L (plane 2), the start of the adjoint's rebuild; checkpoint
         // keeps psi_(a+1) if it is a checkpoint or in the last segment, which the adjoint meets first
@@ -229,6 +238,7 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
                               const float m = MT[p.e];
                               if (KIND == K_FWD) return make_float3(m, 0.f, 0.f);
                               if (A.sset) return make_float3(m, 1.f, 0.f);     // no back-propagator
+                              if constexpr (PFAC) return make_float3(m, 0.f, 0.f);
                               const float2 q = pTb[p.e];
                               return make_float3(m, q.x, q.y);
                           },
@@ -241,7 +251,7 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
                               err += (double)(d * d);
                               if (KIND == K_FWD) return v;
                               const float f = 1.0f - m * __builtin_amdgcn_rsqf(I + 1e-30f);
-                              return cmulc(make_float2(v.x * f, v.y * f), make_float2(o.y, o.z));
+                              return cmulc(make_float2(v.x * f, v.y * f), PFAC && !A.sset ? pfac(p) : make_float2(o.y, o.z));
                           });
         if constexpr (FUSE) {
             // the adjoint's first row pass rides along (none for the error pass)
@@ -391,10 +401,16 @@ __device__ __forceinline__ void process_position(const PosArgs& A, int k, int sl
             else E.inv(NoOp(), post);
         }
         if (a > 0) {
-            auto bypc = staged<16, 32>([&](const Pt& p) { return pTb[p.e]; },
-                                       [&](const Pt&, float2 v, float2 o) { return cmulc(v, o); });
-            if constexpr (FUSE) E.row_step(bypc);
-            else E.fwd(NoOp(), bypc);
+            if constexpr (PFAC) {
+                auto bypc = [&](const Pt& p, float2 v) { return cmulc(v, pfac(p)); };
+                if constexpr (FUSE) E.row_step(bypc);
+                else E.fwd(NoOp(), bypc);
+            } else {
+                auto bypc = staged<16, 32>([&](const Pt& p) { return pTb[p.e]; },
+                                           [&](const Pt&, float2 v, float2 o) { return cmulc(v, o); });
+                if constexpr (FUSE) E.row_step(bypc);
+                else E.fwd(NoOp(), bypc);
+            }
         } else {
             if constexpr (FUSE) E.row_last(NoOp());
             else E.fwd();
@@ -814,8 +830,8 @@ __global__ __launch_bounds__(Cfg<N>::NT, Cfg<N>::MINB) void k_plain(PosArgs A, i
 // nslab positions or a grid that divides nslab, so every slab's positions run on one workgroup in
 // increasing p: each slab sums the same positions in the same order whatever the grid size.
 // SEP: the separable-propagator routine (process_sep; the 256 x 256 engine, stored waves), its 1D factors in LDS.
-// SH: the workgroup shape (Cfg).
-template <int N, int SH, int WV, bool SEP>
+// SH: the workgroup shape (Cfg).  PFAC: the general routine applies the propagator as its 1D factors (LDS).
+template <int N, int SH, int WV, bool SEP, bool PFAC = false>
 __global__ __launch_bounds__((Cfg<N, SH>::NT), (Cfg<N, SH>::MINB)) void k_batch(PosArgs A) {
     constexpr int NT = Cfg<N, SH>::NT;
     MSET_LDS(N, NT, SH)
@@ -829,10 +845,19 @@ __global__ __launch_bounds__((Cfg<N, SH>::NT), (Cfg<N, SH>::MINB)) void k_batch(
             __syncthreads();
         }
     } else {
+        float2* pxs = nullptr;
+        float2* pys = nullptr;
+        if constexpr (PFAC) {
+            __shared__ float2 fx[N], fy[N];
+            for (int i = threadIdx.x; i < 2 * N; i += NT) (i < N ? fx : fy)[i % N] = A.p1d[i];
+            __syncthreads();
+            pxs = fx;
+            pys = fy;
+        }
         for (int p = blockIdx.x; p < A.count; p += gridDim.x) {
             const int k = A.order ? A.order[A.first + p] : A.first + p;
             const int slab = p % A.nslab;
-            process_position<N, NT, K_BATCH, WV>(A, k, blockIdx.x, slab, p < A.nslab, E, phx, phy, red);
+            process_position<N, NT, K_BATCH, WV, PFAC>(A, k, blockIdx.x, slab, p < A.nslab, E, phx, phy, red, pxs, pys);
             __syncthreads();
         }
     }
@@ -924,6 +949,19 @@ __global__ __launch_bounds__(Cfg<N>::NT) void k_plane_fft(const float2* in, floa
         default: return hipErrorInvalidValue;                               \
     }
 
+// the sizes below 256 (the kernels whose plane is on chip)
+#define MSET_DISPATCH_SMALL(N, BODY)                                        \
+    switch (N) {                                                            \
+        case 16: { constexpr int NN = 16; BODY; } break;                    \
+        case 24: { constexpr int NN = 24; BODY; } break;                    \
+        case 32: { constexpr int NN = 32; BODY; } break;                    \
+        case 48: { constexpr int NN = 48; BODY; } break;                    \
+        case 64: { constexpr int NN = 64; BODY; } break;                    \
+        case 96: { constexpr int NN = 96; BODY; } break;                    \
+        case 128: { constexpr int NN = 128; BODY; } break;                  \
+        default: return hipErrorInvalidValue;                               \
+    }
+
 bool plane_size_supported(int n) {
     return n == 16 || n == 24 || n == 32 || n == 48 || n == 64 || n == 96 || n == 128 || n == 256;
 }
@@ -970,6 +1008,8 @@ hipError_t launch_batch(int n, const PosArgs& A, int grid, int shape, hipStream_
         MSET_DISPATCH(n, hipLaunchKernelGGL((k_batch<NN, 0, WAVES_CHECKPOINT, false>), dim3(grid), dim3(Cfg<NN>::NT), 0, st, A))
     } else if (shape) {
         MSET_DISPATCH(n, hipLaunchKernelGGL((k_batch<NN, 1, WAVES_STORE, false>), dim3(grid), dim3(Cfg<NN, 1>::NT), 0, st, A))
+    } else if (A.pfac && n != 256 && !A.sset) {
+        MSET_DISPATCH_SMALL(n, hipLaunchKernelGGL((k_batch<NN, 0, WAVES_STORE, false, true>), dim3(grid), dim3(Cfg<NN>::NT), 0, st, A))
     } else MSET_DISPATCH(n, hipLaunchKernelGGL((k_batch<NN, 0, WAVES_STORE, false>), dim3(grid), dim3(Cfg<NN>::NT), 0, st, A));
     return hipGetLastError();
 }
