@@ -1,0 +1,25 @@
+// batch_variants.hip -- launcher of the batched gradient (k_batch, multislice.hpp) with recomputed or
+// checkpointed exit waves (the default workgroup shape).
+#include "multislice.hpp"
+
+namespace mset {
+
+hipError_t launch_batch_variant(int n, const PosArgs& A, int grid, hipStream_t st) {
+    if (A.waves == WAVES_RECOMPUTE) {
+        if (A.sep) return hipErrorInvalidValue;
+        MSET_DISPATCH(n, hipLaunchKernelGGL((k_batch<NN, 0, WAVES_RECOMPUTE, false>), dim3(grid), dim3(Cfg<NN>::NT), 0, st, A))
+    } else if (A.waves == WAVES_CHECKPOINT) {
+        if (A.sset || A.ckseg < 2) return hipErrorInvalidValue;
+        if (A.sep) {
+            if (n != 256 || A.grad_mode != GRAD_REF) return hipErrorInvalidValue;
+            hipLaunchKernelGGL((k_batch<256, 0, WAVES_CHECKPOINT, true>), dim3(grid), dim3(Cfg<256>::NT), 0, st, A);
+        } else {
+            MSET_DISPATCH(n, hipLaunchKernelGGL((k_batch<NN, 0, WAVES_CHECKPOINT, false>), dim3(grid), dim3(Cfg<NN>::NT), 0, st, A))
+        }
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace mset
