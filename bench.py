@@ -245,10 +245,10 @@ def main():
 
     out = dict(
         metric=METRIC, value=value, unit=UNIT, n_gpus=1, steps=args.steps, warmup=args.warmup,
-        ms_per_step=ms_step, higher_is_better=True, scaling='weak', vs_baseline=None, dtype='f32 (complex64 waves)',
+        ms_per_step=ms_step, higher_is_better=True, scaling='weak', vs_baseline=None, dtype='f32',
         data='synthetic: seeded FCC Gaussian-atom potential, device-simulated Poisson patterns',
         config=dict(workload=WORKLOADS[args.workload],
-                    path='batched', batch=args.batch, hip_graph=use_graph, positions_per_step=wl.positions(),
+                    path='batched', precision='complex64 waves, f64 loss and position-term sums', batch=args.batch, hip_graph=use_graph, positions_per_step=wl.positions(),
                     tilts=ntilt, planes=wl.L, probe=wl.P, pos_slices_per_step=units, step_size=list(STEPS_ALL),
                     autotune=tuned, error_after=err_after, rotate_ms_per_step=ms_r),
         roofline=dict(bound='hbm', achieved=achieved, peak=HBM_PEAK_GBS, unit='GB/s', frac=achieved / HBM_PEAK_GBS,
