@@ -86,8 +86,8 @@ def cpu_baseline(wl, tilt, positions, threads, steps):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=10)
-    ap.add_argument('--warmup', type=int, default=2)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--workload', default='C3', choices=['C2', 'C3'],
                     help='C3 (default): 256^2 probe, 21 tilts with uneven scans; C2: 128^2 probe, 11 tilts')
     ap.add_argument('--batch', type=int, default=None,
