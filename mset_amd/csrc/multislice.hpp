@@ -50,7 +50,7 @@ template <> struct Cfg<16> { static constexpr int NT = 64, MINB = 1;  using Eng 
 template <> struct Cfg<24> { static constexpr int NT = 64, MINB = 1;  using Eng = LdsEngine<24, 64>; };
 template <> struct Cfg<32> { static constexpr int NT = 64, MINB = 1;  using Eng = LdsEngine<32, 64>; };
 template <> struct Cfg<48> { static constexpr int NT = 192, MINB = 1; using Eng = LdsEngine<48, 192>; };
-template <> struct Cfg<64> { static constexpr int NT = 256, MINB = 1; using Eng = LdsEngine<64, 256>; };
+template <> struct Cfg<64> { static constexpr int NT = 256, MINB = 3; using Eng = LdsEngine<64, 256>; };
 template <> struct Cfg<96> { static constexpr int NT = 576, MINB = 1; using Eng = LdsEngine<96, 576>; };
 template <> struct Cfg<128> { static constexpr int NT = 512, MINB = 1; using Eng = RegEngine<128>; };
 template <> struct Cfg<256> { static constexpr int NT = 512, MINB = 4; using Eng = TiledEngine<256, 512>; };
