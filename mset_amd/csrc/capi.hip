@@ -477,17 +477,17 @@ int issue_grad(mset_ctx* c, int first, int count, bool apply, int want) {
     }
     HIPCHK(launch_reduce_slabs(W.gslab.p, used, (size_t)planes(c) * plane(c), W.G.p, c->stream));
     if (sep) HIPCHK(launch_untranspose_odd(W.G.p, c->n, c->L, c->stream));
+    const bool probe_step = apply && c->p.step_size[1] != 0.f && (want & WANT_PROBE);
     if (want & WANT_PROBE) {
-        // in place into slab 0 (each element is read and written by the same thread), then PG = ifftn
+        // in place into slab 0 (each element is read and written by the same thread), then PG = ifftn and,
+        // with a probe step, probe -= s2 PG from the same transform
         HIPCHK(launch_reduce_cslabs(W.pslab.p, used, plane(c), W.pslab.p, c->stream));
-        HIPCHK(launch_plane_fft(c->n, W.pslab.p, W.PG.p, c->tw.p, 1, 1.f, 0.f, W.planes.p, c->stream));
+        HIPCHK(launch_plane_fft(c->n, W.pslab.p, W.PG.p, c->tw.p, 1, 1.f, 0.f, W.planes.p, c->stream,
+                                probe_step ? c->probe.p : nullptr, -c->p.step_size[1], 1.f));
     }
     if (apply) {
         if (c->p.step_size[0] != 0.f)
             HIPCHK(launch_update_volume(c->V.p, W.G.p, c->n, c->nz, binning(c), c->p.step_size[0], c->stream));
-        if (c->p.step_size[1] != 0.f && (want & WANT_PROBE))
-            HIPCHK(launch_plane_fft(c->n, W.pslab.p, c->probe.p, c->tw.p, 1, -c->p.step_size[1], 1.f, W.planes.p,
-                                    c->stream));
         if (c->p.step_size[2] != 0.f && (want & WANT_POS))
             HIPCHK(launch_update_positions(c->x.p + first, c->y.p + first, c->dpos.p + 2 * (size_t)first, count,
                                            c->p.step_size[2], c->stream));

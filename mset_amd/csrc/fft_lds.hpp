@@ -309,6 +309,61 @@ __device__ __forceinline__ void fft2d_inv(float2* s, const float2* tw, const Pre
     fft_pass<N, NT, true, false, 0, 1, false, false, true>(s, tw, pre, post);
 }
 
+// ------------------------------------------------------------------ batched 1D transforms of a tile of lines
+// NLINES lines of N points in LDS, element e of line l at s[l * LS + e * ES]; consecutive lanes take
+// consecutive lines at the same butterfly, so a pitch LS that is odd in float2 (ES = 1) or ES = NLINES
+// with LS = 1 keeps the accesses free of bank conflicts.  Same radix plans and twiddle table as the
+// plane transforms; an inverse transform leaves out the 1/N.
+template <int N, int NT, int NLINES, int ES, int LS, int R, int NS, bool INV>
+__device__ __forceinline__ void line_stage(float2* __restrict__ s, const float2* __restrict__ tw) {
+    constexpr int NL = N / R;
+    constexpr int NB = NLINES * NL;
+    constexpr int CNT = (NB + NT - 1) / NT;
+    constexpr int TWS = N / (NS * R);
+    const int t = threadIdx.x;
+    float2 v[CNT][R];
+#pragma unroll
+    for (int c = 0; c < CNT; ++c) {
+        const int b = t + c * NT;
+        if (NB % NT == 0 || b < NB) {
+            const int line = b % NLINES, j = b / NLINES;   // lanes along lines: conflict-free with odd LS
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[c][r] = s[line * LS + (j + r * NL) * ES];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < CNT; ++c) {
+        const int b = t + c * NT;
+        if (NB % NT == 0 || b < NB) {
+            const int line = b % NLINES, j = b / NLINES;
+            const int k = j % NS;
+            if constexpr (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    float2 w = tw[k * r * TWS];
+                    if (INV) w.y = -w.y;
+                    v[c][r] = cmul(v[c][r], w);
+                }
+            }
+            DFT<R, INV>::run(v[c]);
+            const int base = (j / NS) * NS * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) s[line * LS + (base + r * NS) * ES] = v[c][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <int N, int NT, int NLINES, int ES, int LS, int ST, int NS, bool INV>
+__device__ __forceinline__ void line_fft(float2* s, const float2* tw) {
+    if constexpr (ST < Plan<N>::S) {
+        constexpr int R = Plan<N>::R[ST];
+        line_stage<N, NT, NLINES, ES, LS, R, NS, INV>(s, tw);
+        line_fft<N, NT, NLINES, ES, LS, ST + 1, NS * R, INV>(s, tw);
+    }
+}
+
 // The LDS-stage transforms behind the engine interface shared with the register-resident engine:
 // fwd / inv with (y, x, value) functors, each_R / each_C to visit the plane after a forward / inverse
 // transform (lanes along y, resp. along x, so [x][y] resp. [y][x] global arrays stay coalesced).

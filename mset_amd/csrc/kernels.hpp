@@ -87,8 +87,12 @@ hipError_t launch_batch(int n, const PosArgs& A, int grid, int shape, hipStream_
 hipError_t launch_fwd(int n, const PosArgs& A, int grid, hipStream_t st);
 // forward + loss over the work items A.order[0 .. A.count) of several tilts (A.tilts)
 hipError_t launch_fwd_tilts(int n, const PosArgs& A, int grid, hipStream_t st);
+// 2D transform of one plane: dir 0 forward, in [y][x] -> out [x][y]; dir 1 inverse, in [x][y] -> out [y][x]
+// = alpha/N^2 v + beta out, and with out2 also out2 = alpha2/N^2 v + beta2 out2.  planes: at N = 256 one
+// plane of scratch (the transform runs as two passes of line transforms over the device).
 hipError_t launch_plane_fft(int n, const float2* in, float2* out, const float2* tw, int dir, float alpha, float beta,
-                            float2* planes, hipStream_t st);
+                            float2* planes, hipStream_t st, float2* out2 = nullptr, float alpha2 = 0.f,
+                            float beta2 = 0.f);
 bool plane_in_global(int n);     // the position kernels keep the plane in global memory (planes buffer)
 hipError_t occupancy_batch(int n, int shape, int* blocks_per_cu);
 

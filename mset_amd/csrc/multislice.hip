@@ -46,9 +46,27 @@ hipError_t launch_fwd_tilts(int n, const PosArgs& A, int grid, hipStream_t st) {
 }
 
 hipError_t launch_plane_fft(int n, const float2* in, float2* out, const float2* tw, int dir, float alpha, float beta,
-                            float2* planes, hipStream_t st) {
-    MSET_DISPATCH(n, hipLaunchKernelGGL(k_plane_fft<NN>, dim3(1), dim3(Cfg<NN>::NT), 0, st, in, out, tw, dir, alpha, beta,
-                                        planes));
+                            float2* planes, hipStream_t st, float2* out2, float alpha2, float beta2) {
+    if (n == 256) {
+        // rows of `in` into the scratch plane, then its columns into `out`: 2 x 64 workgroups
+        constexpr int N = 256, TL = 4, NT = 64;
+        if (!planes) return hipErrorInvalidValue;
+        if (dir == 0) {
+            hipLaunchKernelGGL((k_plane_lines<N, TL, NT, false, true>), dim3(N / TL), dim3(NT), 0, st, in, planes, tw,
+                               1.f, 0.f, nullptr, 0.f, 0.f);
+            hipLaunchKernelGGL((k_plane_lines<N, TL, NT, false, false>), dim3(N / TL), dim3(NT), 0, st, planes, out, tw,
+                               1.f, 0.f, nullptr, 0.f, 0.f);
+        } else {
+            hipLaunchKernelGGL((k_plane_lines<N, TL, NT, true, true>), dim3(N / TL), dim3(NT), 0, st, in, planes, tw,
+                               1.f, 0.f, nullptr, 0.f, 0.f);
+            hipLaunchKernelGGL((k_plane_lines<N, TL, NT, true, false>), dim3(N / TL), dim3(NT), 0, st, planes, out, tw,
+                               alpha, beta, out2, alpha2, beta2);
+        }
+        return hipGetLastError();
+    }
+    // smaller planes: one workgroup transforms the whole plane on chip
+    MSET_DISPATCH_SMALL(n, hipLaunchKernelGGL(k_plane_fft<NN>, dim3(1), dim3(Cfg<NN>::NT), 0, st, in, out, tw, dir, alpha,
+                                              beta, planes, out2, alpha2, beta2));
     return hipGetLastError();
 }
 

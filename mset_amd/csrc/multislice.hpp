@@ -911,7 +911,8 @@ __global__ __launch_bounds__(Cfg<N>::NT, Cfg<N>::MINB) void k_fwd(PosArgs A) {
 //   dir 1: out = beta * out + alpha * ifftn(in) (with 1/N^2), in transposed ([x][y]), out row-major.
 template <int N>
 __global__ __launch_bounds__(Cfg<N>::NT) void k_plane_fft(const float2* in, float2* out, const float2* twg, int dir,
-                                                         float alpha, float beta, float2* planes) {
+                                                         float alpha, float beta, float2* planes, float2* out2,
+                                                         float alpha2, float beta2) {
     constexpr int NT = Cfg<N>::NT;
     using Eng = typename Cfg<N>::Eng;
     __shared__ float2 s[Eng::LDS_ELEMS];
@@ -929,13 +930,62 @@ __global__ __launch_bounds__(Cfg<N>::NT) void k_plane_fft(const float2* in, floa
                              });
     } else {
         const BufC ib(in, N * N), ob(out, N * N);           // in [x][y], out [y][x]
-        const float sc = alpha / (float)(N * N);
+        const BufC ob2(out2, out2 ? N * N : 0);
+        const float sc = alpha / (float)(N * N), sc2 = alpha2 / (float)(N * N);
         E.template inv<true>([&](const Pt& p, float2) { return ib[p.e]; },
                              [&](const Pt& p, float2 v) {
                                  const float2 w = cscale(v, sc);
                                  ob.put(p.e, beta == 0.f ? w : cadd(cscale(ob[p.e], beta), w));
+                                 if (out2) {
+                                     const float2 w2 = cscale(v, sc2);
+                                     ob2.put(p.e, beta2 == 0.f ? w2 : cadd(cscale(ob2[p.e], beta2), w2));
+                                 }
                                  return v;
                              });
+    }
+}
+
+// One pass of the 2D transform of an N x N plane as batched line transforms, TL lines per workgroup:
+// at N = 256 the plane does not fit in one workgroup's LDS, and a one-workgroup transform streams it
+// through a single CU.  ROWS: transform the rows of `in` into `out` (same layout).  Otherwise transform
+// the columns of `in` and store column j as row j of `out` (the transposed layouts of launch_plane_fft),
+// with out = alpha/N^2 v + beta out for an inverse and the same into out2 with (alpha2, beta2).  Both
+// passes keep line l at s[l (N + 1) + e]; loads and stores run along rows of the global arrays.
+template <int N, int TL, int NT, bool INV, bool ROWS>
+__global__ __launch_bounds__(NT) void k_plane_lines(const float2* __restrict__ in, float2* __restrict__ out,
+                                                    const float2* __restrict__ twg, float alpha, float beta,
+                                                    float2* __restrict__ out2, float alpha2, float beta2) {
+    constexpr int LS = N + 1;
+    __shared__ float2 s[LS * TL];
+    __shared__ float2 tw[N];
+    for (int i = threadIdx.x; i < N; i += NT) tw[i] = twg[i];
+    const int l0 = blockIdx.x * TL;
+    for (int i = threadIdx.x; i < N * TL; i += NT) {
+        if (ROWS) {
+            const int l = i / N, e = i % N;
+            s[l * LS + e] = in[(size_t)(l0 + l) * N + e];
+        } else {
+            const int l = i % TL, e = i / TL;
+            s[l * LS + e] = in[(size_t)e * N + l0 + l];
+        }
+    }
+    __syncthreads();
+    line_fft<N, NT, TL, 1, LS, 0, 1, INV>(s, tw);
+    const float sc = alpha / (float)(N * N), sc2 = alpha2 / (float)(N * N);
+    for (int i = threadIdx.x; i < N * TL; i += NT) {
+        const int l = i / N, e = i % N;
+        const float2 v = s[l * LS + e];
+        const size_t g = (size_t)(l0 + l) * N + e;
+        if (ROWS || !INV) {
+            out[g] = v;
+        } else {
+            const float2 w = cscale(v, sc);
+            out[g] = beta == 0.f ? w : cadd(cscale(out[g], beta), w);
+            if (out2) {
+                const float2 w2 = cscale(v, sc2);
+                out2[g] = beta2 == 0.f ? w2 : cadd(cscale(out2[g], beta2), w2);
+            }
+        }
     }
 }
 
