@@ -164,16 +164,24 @@ def main():
         args.batch = batches[0]
     log('batched path: batch %d, graph %s, steps %s' % (args.batch, use_graph, STEPS_ALL))
 
-    def timed(nsteps, nwarm):
+    def timed(nsteps, nwarm, marks=None):
+        # every iterate call returns after its pass has finished on the device (it synchronises the stream),
+        # so the time between two returns is one step
         for _ in range(nwarm):
             r.iterate(lib.PATH_BATCHED, args.batch, use_graph=use_graph, want_error=False)
         t0 = time.perf_counter()
+        if marks is not None:
+            marks.append(t0)
         for _ in range(nsteps):
             r.iterate(lib.PATH_BATCHED, args.batch, use_graph=use_graph, want_error=False)
+            if marks is not None:
+                marks.append(time.perf_counter())
         return time.perf_counter() - t0
 
-    dt = timed(args.steps, args.warmup)
+    marks = []
+    dt = timed(args.steps, args.warmup, marks)
     ms_step = dt / args.steps * 1e3
+    step_ms = np.diff(marks) * 1e3
     value = units * args.steps / dt
     err_after = float(np.mean(r.error_pass()))
     log('timed: %.2f ms per step, %.0f pos-slices/s' % (ms_step, value))
@@ -245,7 +253,8 @@ def main():
 
     out = dict(
         metric=METRIC, value=value, unit=UNIT, n_gpus=1, steps=args.steps, warmup=args.warmup,
-        ms_per_step=ms_step, higher_is_better=True, scaling='weak', vs_baseline=None, dtype='f32',
+        ms_per_step=ms_step, ms_per_step_median=float(np.median(step_ms)),
+        ms_per_step_range=[float(step_ms.min()), float(step_ms.max())], higher_is_better=True, scaling='weak', vs_baseline=None, dtype='f32',
         data='synthetic: seeded FCC Gaussian-atom potential, device-simulated Poisson patterns',
         config=dict(workload=WORKLOADS[args.workload],
                     path='batched', precision='complex64 waves, f64 loss and position-term sums', batch=args.batch, hip_graph=use_graph, positions_per_step=wl.positions(),

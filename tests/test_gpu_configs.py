@@ -1,8 +1,9 @@
 """GPU parity at the named configurations: C1 (the reference's CPU-runnable case, one update pass over all
-tilts), C2 (a slot-reused tilt against the oracle; full-size repeatability, additivity and graph replay), C3
-(the full series' graph-captured pass against the eager one; FP32 device gradient against the FP64 oracle
-at the 256^2, 32-plane shape) and C4 (device gradients against central finite differences of the FP64 loss
-along 50 seeded directions, h = 1e-4; stored, recomputed and checkpointed waves at 256^3)."""
+tilts), C2 (the plain path and a slot-reused batched tilt against the oracle; full-size repeatability,
+additivity and graph replay), C3 (the full series' graph-captured pass against the eager one; FP32 device
+gradient against the FP64 oracle at the 256^2, 32-plane shape) and C4 (device gradients against central
+finite differences of the FP64 loss along 50 seeded directions, h = 1e-4; stored, recomputed and
+checkpointed waves at 256^3)."""
 import numpy as np
 import pytest
 
@@ -76,6 +77,40 @@ def test_c2_tilt_batched_with_slot_reuse_vs_oracle():
     print('C2 (32 positions on 8 slots) FP32 vs FP64 relative errors', r, flush=True)
     assert r['waves'] < 1e-4 and r['loss'] < 1e-4
     assert r['dV'] < 1e-3 and r['dprobe'] < 1e-3 and r['dpos'] < 1e-3
+
+
+def test_c2_tilt_plain_vs_oracle():
+    """The C2 configuration's own path: one position per launch with the volume, probe and positions updated
+    after every position (Run_MSET_recon.m:134-158), at the C2 shape (128 x 128, 16 planes), over 24 positions
+    spread across a C2 tilt, from half the true potential, with the bench's three nonzero steps, against the
+    FP64 oracle's sequential loop."""
+    from mset_amd import lib
+    from tests.test_gpu_parity import assert_update_close
+    ctx = lib.Context(0)
+    wl = _workload('C2', ctx, tilts=[2])
+    k = np.arange(0, wl.npos_of(0), wl.npos_of(0) // 24)[:24]
+    V = (0.5 * wl.truth).astype(np.float64)
+    xl, yl, M = wl.xlists[0][k], wl.ylists[0][k], wl.measured[0][k]
+    steps = (100.0, 1e-3, 10.0)
+    print('case: C2 plain, %d positions, steps %s' % (len(k), steps), flush=True)
+    ctx.upload(V, wl.probe, wl.prop, xl, yl, M, wl.sigma, wl.dx, slice_binning=wl.binning, step_size=steps,
+               n_scan=(len(k), 1))
+    nV, npr, nxl, nyl, err = ctx.run()
+    ctx.close()
+    oV, opr, oxl, oyl, oerr, _ = O.run_tilt(V, wl.probe, wl.prop, np.conj(wl.prop), xl, yl, M, len(k), 1,
+                                            wl.binning, wl.sigma, wl.dx, steps)
+    d, od = np.c_[nxl - xl, nyl - yl], np.c_[oxl - xl, oyl - yl]
+    print('C2 plain: loss %.3e, V update %.2e, probe update %.2e, position moves %.2e (largest %.3f px)'
+          % (abs(err / oerr - 1), rel(nV - V, oV - V), rel(npr - wl.probe, opr - wl.probe), rel(d, od),
+             np.abs(od).max()), flush=True)
+    assert abs(err / oerr - 1) < 1e-4
+    # from half the potential the 24 object steps move V by about three times its norm, so the volume is
+    # compared through its update (measured 3.4e-5); the probe moves by 0.2 % and is compared both ways
+    assert rel(nV - V, oV - V) < 1e-3
+    assert_update_close(npr, opr, wl.probe, len(k))
+    ulp = np.spacing(np.float32(max(np.abs(xl).max(), np.abs(yl).max())))
+    assert np.abs(od).max() > 10 * ulp                      # the position step moved the positions
+    assert np.abs(d - od).max() <= max(1e-3 * np.abs(od).max(), 4 * ulp)
 
 
 def test_c2_full_size_properties():
