@@ -277,6 +277,28 @@ def test_single_plane_and_single_position_edges(mset, n, nz, binning, npos, mode
     assert_update_close(p2, op, c['probe'], npos)
 
 
+@pytest.mark.parametrize('n', [64, 128, 256])
+def test_positions_on_and_beyond_the_field_border(mset, n):
+    """Scan positions on the field border and up to a field width outside it.  The sub-pixel shift is a
+    circular Fourier shift (Func_generate_probe_wave_scan_sub_pixel.m:17-25), so the probe wraps around the
+    field; the device's phase ramp, its position terms and the waves must follow the oracle for any shift."""
+    c = small_case(n=n, nz=4, binning=2, npos=6, seed=5 * n, spread=1.0)
+    c['xl'] = np.array([1.0, n, n + 5.3, -3.7, 0.5, 2 * n - 1.25], np.float32)
+    c['yl'] = np.array([n, 1.0, -2.6, n + 7.9, n / 2 + 0.25, 3.5 - n], np.float32)
+    print('case: border positions', n, flush=True)
+    _upload_case(mset, c)
+    dV, dp, dpos, err = mset.grad()
+    odV, odp, odpos, oep = O.grad_fixed(c['V'], c['probe'], c['prop'], c['bprop'], c['xl'], c['yl'], c['M'],
+                                        c['bin'], c['sig'], c['dx'])
+    assert abs(err / (oep.sum() / c['npos']) - 1) < LOSS_TOL
+    assert rel(dV, odV) < GRAD_TOL and rel(dp, odp) < GRAD_TOL and rel(dpos, odpos) < GRAD_TOL
+    for k in (2, 3, 5):
+        w = mset.waves(k)
+        psi0 = O.subpixel_shift(c['probe'], c['xl'][k], c['yl'][k])
+        save, _, _, _ = O.forward(c['V'], psi0, c['prop'], c['M'][k], c['bin'], c['sig'], c['dx'])
+        assert rel(w[0], psi0) < WAVE_TOL and rel(w[-1], save[-1]) < WAVE_TOL, k
+
+
 def test_upload_from_page_locked_host_memory(mset):
     """Patterns staged in mset_host_alloc memory upload to the same device state as pageable ones."""
     from mset_amd import lib
