@@ -9,18 +9,24 @@
 // other than its shear axis, a phase ramp, and the inverse transform; the real part is taken after
 // the fourth shear of every step.  Transforms along different axes commute and nothing but the phase
 // ramps happens between shears, so the engine tracks which axes are currently in the Fourier domain
-// and only transforms the axes whose domain must change: a step costs 10 one-axis passes instead of
-// 16, and each shear's phase ramp is applied inside the pass that completes its domain change.
+// and only transforms the axes whose domain must change: a step costs 10 one-axis transforms instead
+// of 16, and each shear's phase ramp is applied right after the transform that completes its domain
+// change.  The transforms of one domain change may run in any order; they are ordered so that a shear
+// starts on the axis the previous one ended on, and those two transforms share one pass over the
+// volume (7 passes per step instead of 10 for the shear axes of a tilt about y).
 //
-// A pass is a batched 1D FFT along one axis of the complex work volume: each workgroup loads a tile
-// of TL lines into LDS (for the y and z axes a tile is TL consecutive x positions, so loads stay
-// coalesced), runs the Stockham stages of fft_lds.hpp's radix plans with consecutive lanes on
-// consecutive lines (so LDS accesses are free of bank conflicts), and writes the tile back with
-// the 1/N of an inverse transform, the phase ramp and the real part fused into the store.
+// A pass is a batched 1D FFT along one axis of the complex work volume, or two FFTs of the same lines:
+// each workgroup loads a tile of TL lines into LDS (for the y and z axes a tile is TL consecutive x
+// positions, so loads stay coalesced), runs the Stockham stages of fft_lds.hpp's radix plans with
+// consecutive lanes on consecutive lines (so LDS accesses are free of bank conflicts), and writes the
+// tile back with the 1/N of an inverse transform, the phase ramp and the real part fused into the store
+// (between two transforms, into an LDS sweep).
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cmath>
 #include <string>
+#include <vector>
 
 #include "../../include/mset.h"
 #include "fft_lds.hpp"
@@ -39,13 +45,19 @@ int rfail(int code, const std::string& m) {
 const char* rot_last_error() { return r_err.c_str(); }
 
 // ------------------------------------------------------------------ device side
-struct PassArgs {
-    float2* w;
-    int nx, ny, nz;
+// One transform of the lines and the element-wise work that follows it
+struct LineOp {
     int inv;            // 1: inverse transform (with 1/N)
     int phase_dim;      // 0: none; 1, 2, 3: shear along x, y, z with coefficients (a, b)
     double a, b;
-    int take_real;      // store the real part only
+    int take_real;      // keep the real part only
+};
+
+struct PassArgs {
+    float2* w;
+    int nx, ny, nz;
+    int nops;           // 1, or 2: a second transform of the same lines while the tile is in LDS
+    LineOp op[2];
     const float2* tw;
 };
 
@@ -55,15 +67,15 @@ __device__ __forceinline__ double shear_grid(int j, int n) {
     return (double)(((j + n / 2) % n) + 1 - (int)floor((n + 1) / 2.0 + 0.5));
 }
 
-__device__ __forceinline__ float2 shear_phase(const PassArgs& P, int jx, int jy, int jz) {
+__device__ __forceinline__ float2 shear_phase(const PassArgs& P, const LineOp& o, int jx, int jy, int jz) {
     // even-size Nyquist planes of any axis zero all three grids (:115-129) -> phase 1
     if ((P.nx % 2 == 0 && jx == P.nx / 2) || (P.ny % 2 == 0 && jy == P.ny / 2) || (P.nz % 2 == 0 && jz == P.nz / 2))
         return make_float2(1.f, 0.f);
     const double gx = shear_grid(jx, P.nx), gy = shear_grid(jy, P.ny), gz = shear_grid(jz, P.nz);
     double arg;   // SheerTransforms_for3Drot.m:29-48, exp(1i*2*pi*arg)
-    if (P.phase_dim == 1) arg = (P.a * gy + P.b * gz) * gx / P.nx;
-    else if (P.phase_dim == 2) arg = (P.b * gx + P.a * gz) * gy / P.ny;
-    else arg = (P.a * gx + P.b * gy) * gz / P.nz;
+    if (o.phase_dim == 1) arg = (o.a * gy + o.b * gz) * gx / P.nx;
+    else if (o.phase_dim == 2) arg = (o.b * gx + o.a * gz) * gy / P.ny;
+    else arg = (o.a * gx + o.b * gy) * gz / P.nz;
     const float f = (float)(arg - rint(arg));
     float sn, cs;
     sincospif(2.f * f, &sn, &cs);
@@ -119,19 +131,35 @@ __global__ __launch_bounds__(NT) void k_axis_pass(PassArgs P) {
         s[l * LS + e * ES] = ok ? P.w[g] : make_float2(0.f, 0.f);
     }
     __syncthreads();
-    if (P.inv) line_fft<N, NT, TL, ES, LS, 0, 1, true>(s, tw);
+    // the inverse transform's 1/N, the shear phase and the real part after a transform
+    auto finish = [&](const LineOp& o, float2 v, int jx, int jy, int jz) {
+        if (o.inv) v = cscale(v, 1.0f / (float)N);
+        if (o.phase_dim) v = cmul(v, shear_phase(P, o, jx, jy, jz));
+        if (o.take_real) v.y = 0.f;
+        return v;
+    };
+    if (P.op[0].inv) line_fft<N, NT, TL, ES, LS, 0, 1, true>(s, tw);
     else line_fft<N, NT, TL, ES, LS, 0, 1, false>(s, tw);
-    const float sc = P.inv ? 1.0f / (float)N : 1.0f;
+    if (P.nops > 1) {
+        for (int i = threadIdx.x; i < N * TL; i += NT) {
+            const int l = AXIS == 0 ? i / N : i % TL, e = AXIS == 0 ? i % N : i / TL;
+            bool ok;
+            int jx, jy, jz;
+            (void)gidx(l, e, ok, jx, jy, jz);
+            if (ok) s[l * LS + e * ES] = finish(P.op[0], s[l * LS + e * ES], jx, jy, jz);
+        }
+        __syncthreads();
+        if (P.op[1].inv) line_fft<N, NT, TL, ES, LS, 0, 1, true>(s, tw);
+        else line_fft<N, NT, TL, ES, LS, 0, 1, false>(s, tw);
+    }
+    const LineOp last = P.nops > 1 ? P.op[1] : P.op[0];
     for (int i = threadIdx.x; i < N * TL; i += NT) {
         const int l = AXIS == 0 ? i / N : i % TL, e = AXIS == 0 ? i % N : i / TL;
         bool ok;
         int jx, jy, jz;
         const size_t g = gidx(l, e, ok, jx, jy, jz);
         if (!ok) continue;
-        float2 v = cscale(s[l * LS + e * ES], sc);
-        if (P.phase_dim) v = cmul(v, shear_phase(P, jx, jy, jz));
-        if (P.take_real) v.y = 0.f;
-        P.w[g] = v;
+        P.w[g] = finish(last, s[l * LS + e * ES], jx, jy, jz);
     }
 }
 
@@ -325,6 +353,117 @@ static float2* twiddle_table(RotWork& W, int n, hipStream_t st, hipError_t& e) {
     return d;
 }
 
+namespace {
+
+struct AxisOp {
+    int axis;
+    LineOp op;
+};
+
+// The axes one domain change transforms (in any order) and what follows the change: the shear phase of a
+// shear, or the real part at the end of a step.  Both belong to the last transform of the group.
+struct LineGroup {
+    int n = 0;
+    int axis[3];
+    bool inv[3];
+    int phase_dim = 0;
+    double a = 0, b = 0;
+    bool take_real = false;
+};
+
+// Func_rotate_3Dvol_ROTmat_FourierShear.m:155-166: every step runs its four shears, each a transform over
+// the two axes other than its shear axis, then returns to real space and keeps the real part.  Transforms
+// along different axes commute, so a shear only transforms the axes whose domain must change.
+bool shear_groups(const ShearPlan& P, std::vector<LineGroup>& out) {
+    bool fourier[3] = {false, false, false};
+    for (int s = 0; s < P.num_step; ++s) {
+        for (int k = 0; k < 4; ++k) {
+            LineGroup g;
+            const int S = P.dims[k] - 1;
+            for (int ax = 0; ax < 3; ++ax)
+                if (fourier[ax] != (ax != S)) {
+                    g.axis[g.n] = ax;
+                    g.inv[g.n++] = fourier[ax];
+                    fourier[ax] = !fourier[ax];
+                }
+            if (g.n == 0) return false;
+            g.phase_dim = P.dims[k];
+            g.a = P.ab[k][0];
+            g.b = P.ab[k][1];
+            out.push_back(g);
+        }
+        LineGroup g;   // back to real space, real part (:160, :165)
+        for (int ax = 0; ax < 3; ++ax)
+            if (fourier[ax]) {
+                g.axis[g.n] = ax;
+                g.inv[g.n++] = true;
+                fourier[ax] = false;
+            }
+        g.take_real = true;
+        if (g.n) out.push_back(g);
+    }
+    return true;
+}
+
+// Orders the transforms of each group so that as many groups as possible start on the axis the previous
+// group ended on (a dynamic programme over the last axis); such neighbours share one pass over the volume.
+// With the shear axes (z, x, y, z) of a tilt about y, a step takes 7 passes instead of 10.
+std::vector<AxisOp> order_ops(const std::vector<LineGroup>& G) {
+    const int ng = (int)G.size();
+    static const int perm2[2][2] = {{0, 1}, {1, 0}};
+    static const int perm3[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+    auto nperm = [&](int g) { return G[g].n == 1 ? 1 : G[g].n == 2 ? 2 : 6; };
+    auto at = [&](int g, int p, int i) { return G[g].n == 1 ? 0 : G[g].n == 2 ? perm2[p][i] : perm3[p][i]; };
+    // best[g][a]: most shared passes over groups 0..g when group g ends on axis a; order[g][a]: the order of
+    // group g that does it; prev[g][a]: the axis group g - 1 then ends on
+    std::vector<std::array<int, 3>> best(ng), order(ng), prev(ng);
+    for (int g = 0; g < ng; ++g) {
+        best[g] = {-1, -1, -1};
+        for (int p = 0; p < nperm(g); ++p) {
+            const int first = G[g].axis[at(g, p, 0)], last = G[g].axis[at(g, p, G[g].n - 1)];
+            int score = 0, from = -1;
+            if (g > 0) {
+                score = -1;
+                for (int a = 0; a < 3; ++a)
+                    if (best[g - 1][a] >= 0 && best[g - 1][a] + (a == first) > score) {
+                        score = best[g - 1][a] + (a == first);
+                        from = a;
+                    }
+            }
+            if (score > best[g][last]) {
+                best[g][last] = score;
+                order[g][last] = p;
+                prev[g][last] = from;
+            }
+        }
+    }
+    std::vector<int> perm(ng);
+    int a = 0;
+    for (int b = 1; b < 3; ++b)
+        if (best[ng - 1][b] > best[ng - 1][a]) a = b;
+    for (int g = ng - 1; g >= 0; --g) {
+        perm[g] = order[g][a];
+        a = prev[g][a];
+    }
+    std::vector<AxisOp> ops;
+    for (int g = 0; g < ng; ++g)
+        for (int i = 0; i < G[g].n; ++i) {
+            const int j = at(g, perm[g], i);
+            const bool last = i == G[g].n - 1;
+            AxisOp o;
+            o.axis = G[g].axis[j];
+            o.op.inv = G[g].inv[j] ? 1 : 0;
+            o.op.phase_dim = last ? G[g].phase_dim : 0;
+            o.op.a = last ? G[g].a : 0.0;
+            o.op.b = last ? G[g].b : 0.0;
+            o.op.take_real = last && G[g].take_real ? 1 : 0;
+            ops.push_back(o);
+        }
+    return ops;
+}
+
+}  // namespace
+
 int rot_run(RotWork& W, float* vol, int nx, int ny, int nz, const double R[9], double rot_step_init, hipStream_t st) {
     if (!rot_size_supported(nx) || !rot_size_supported(ny) || !rot_size_supported(nz))
         return rfail(MSET_EUNSUPPORTED, "rotation line length not built");
@@ -350,42 +489,22 @@ int rot_run(RotWork& W, float* vol, int nx, int ny, int nz, const double R[9], d
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return rfail(MSET_EDEVICE, std::string("rotation load: ") + hipGetErrorString(e));
 
-    bool fourier[3] = {false, false, false};
-    auto pass = [&](int axis, bool inv, int phase_dim, double a, double b, bool take_real) -> hipError_t {
+    std::vector<LineGroup> groups;
+    if (!shear_groups(P, groups)) return rfail(MSET_EDEVICE, "shear plan repeats an axis");
+    const std::vector<AxisOp> ops = order_ops(groups);
+    for (size_t i = 0; i < ops.size();) {
+        // a transform of the same lines right after this one runs in the same pass
+        const bool two = i + 1 < ops.size() && ops[i + 1].axis == ops[i].axis;
         PassArgs A{};
         A.w = W.w; A.nx = nx; A.ny = ny; A.nz = nz;
-        A.inv = inv ? 1 : 0;
-        A.phase_dim = phase_dim; A.a = a; A.b = b;
-        A.take_real = take_real ? 1 : 0;
+        A.nops = two ? 2 : 1;
+        A.op[0] = ops[i].op;
+        if (two) A.op[1] = ops[i + 1].op;
+        const int axis = ops[i].axis;
         A.tw = tw[axis];
-        fourier[axis] = !inv;
-        if (axis == 0) return launch_pass<0>(nx, A, st);
-        if (axis == 1) return launch_pass<1>(ny, A, st);
-        return launch_pass<2>(nz, A, st);
-    };
-    for (int s = 0; s < P.num_step; ++s) {
-        for (int k = 0; k < 4; ++k) {
-            const int S = P.dims[k] - 1;
-            int todo[3], nt = 0;
-            for (int ax = 0; ax < 3; ++ax) {
-                const bool want_f = ax != S;
-                if (fourier[ax] != want_f) todo[nt++] = ax;
-            }
-            for (int i = 0; i < nt; ++i) {
-                const bool last = i == nt - 1;
-                e = pass(todo[i], fourier[todo[i]], last ? P.dims[k] : 0, P.ab[k][0], P.ab[k][1], false);
-                if (e != hipSuccess) return rfail(MSET_EDEVICE, std::string("shear pass: ") + hipGetErrorString(e));
-            }
-            if (nt == 0) return rfail(MSET_EDEVICE, "shear plan repeats an axis");
-        }
-        // back to real space, real part (:160, :165)
-        int todo[3], nt = 0;
-        for (int ax = 0; ax < 3; ++ax)
-            if (fourier[ax]) todo[nt++] = ax;
-        for (int i = 0; i < nt; ++i) {
-            e = pass(todo[i], true, 0, 0.0, 0.0, i == nt - 1);
-            if (e != hipSuccess) return rfail(MSET_EDEVICE, std::string("shear pass: ") + hipGetErrorString(e));
-        }
+        e = axis == 0 ? launch_pass<0>(nx, A, st) : axis == 1 ? launch_pass<1>(ny, A, st) : launch_pass<2>(nz, A, st);
+        if (e != hipSuccess) return rfail(MSET_EDEVICE, std::string("shear pass: ") + hipGetErrorString(e));
+        i += two ? 2 : 1;
     }
     hipLaunchKernelGGL(k_rot_store, dim3(g), dim3(256), 0, st, W.w, vol, nx, ny, nz);
     e = hipGetLastError();
