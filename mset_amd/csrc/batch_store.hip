@@ -8,11 +8,13 @@ namespace mset {
 hipError_t launch_batch_variant(int n, const PosArgs& A, int grid, hipStream_t st);
 
 hipError_t launch_batch(int n, const PosArgs& A, int grid, int shape, hipStream_t st) {
-    if (shape != 0 && (shape != 1 || A.waves != WAVES_STORE)) return hipErrorInvalidValue;
+    if (shape != 0 && (shape > 2 || A.waves != WAVES_STORE)) return hipErrorInvalidValue;
+    if (shape == 2 && (n != 256 || !A.sep)) return hipErrorInvalidValue;   // the wide shape: separable 256 only
     if (A.waves != WAVES_STORE) return launch_batch_variant(n, A, grid, st);
     if (A.sep) {
         if (n != 256 || A.grad_mode != GRAD_REF || A.sset) return hipErrorInvalidValue;
-        if (shape) hipLaunchKernelGGL((k_batch<256, 1, WAVES_STORE, true>), dim3(grid), dim3(Cfg<256, 1>::NT), 0, st, A);
+        if (shape == 2) hipLaunchKernelGGL((k_batch<256, 2, WAVES_STORE, true>), dim3(grid), dim3(Cfg<256, 2>::NT), 0, st, A);
+        else if (shape) hipLaunchKernelGGL((k_batch<256, 1, WAVES_STORE, true>), dim3(grid), dim3(Cfg<256, 1>::NT), 0, st, A);
         else hipLaunchKernelGGL((k_batch<256, 0, WAVES_STORE, true>), dim3(grid), dim3(Cfg<256>::NT), 0, st, A);
     } else if (shape) {
         MSET_DISPATCH(n, hipLaunchKernelGGL((k_batch<NN, 1, WAVES_STORE, false>), dim3(grid), dim3(Cfg<NN, 1>::NT), 0, st, A))

@@ -2,7 +2,7 @@
 //
 // A 256 x 256 complex plane (512 KB) fits neither one CU's LDS nor its registers, so the plane stays in
 // two per-workgroup global buffers and every 1D pass streams it through LDS in tiles of TL lines (TL = NT / 16:
-// 32 with 512 threads, 16 with 256):
+// 32 with 512 threads, 16 with 256, 64 with 1024):
 //   RS  row-major [y][x]     the state after an inverse transform (C orientation, lines are columns x)
 //   XS  transposed [x][y]    the state after a forward transform  (R orientation, lines are rows y)
 // A 256-point line transform is two radix-16 Stockham stages, one butterfly per thread per stage
@@ -84,7 +84,7 @@ template <int N, int NT_> struct TiledEngine {
     static constexpr int TP = TL + 1;            // LDS pitch
     static constexpr int LDS_ELEMS = N * TP;
     static constexpr bool FUSED = true;          // provides the fused passes below
-    static_assert(N == 256 && NT == TL * (N / R) && (TL == 16 || TL == 32), "one butterfly per thread per stage");
+    static_assert(N == 256 && NT == TL * (N / R) && (TL == 16 || TL == 32 || TL == 64), "one butterfly per thread per stage");
     enum { NONE = 0, FWD = 1, INV = 2 };
     float2* s;
     const float2* tw;

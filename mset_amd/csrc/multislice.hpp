@@ -62,6 +62,9 @@ template <> struct Cfg<64, 1> { static constexpr int NT = 512, MINB = 1; using E
 template <> struct Cfg<96, 1> { static constexpr int NT = 1024, MINB = 1; using Eng = LdsEngine<96, 1024>; };
 template <> struct Cfg<128, 1> { static constexpr int NT = 512, MINB = 4; using Eng = RegEngine<128, true>; };
 template <> struct Cfg<256, 1> { static constexpr int NT = 256, MINB = 3; using Eng = TiledEngine<256, 256>; };
+// launches of at most one position per CU (a tilt's last mini-batch): 64-line tiles, one workgroup of 16 waves
+// per CU, twice the default workgroup's loads in flight
+template <> struct Cfg<256, 2> { static constexpr int NT = 1024, MINB = 4; using Eng = TiledEngine<256, 1024>; };
 
 
 enum { K_PLAIN = 0, K_BATCH = 1, K_FWD = 2 };

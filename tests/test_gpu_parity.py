@@ -335,6 +335,23 @@ def test_positions_on_and_beyond_the_field_border(mset, n):
         assert rel(w[0], psi0) < WAVE_TOL and rel(w[-1], save[-1]) < WAVE_TOL, k
 
 
+def test_separable_256_launch_above_one_position_per_cu(mset):
+    """At 256 x 256 a launch of at most one position per CU runs the separable routine's 1024-thread shape,
+    a longer one the default 512-thread shape.  The other 256 cases are short; this one (264 positions)
+    runs the default shape in one launch and the 1024-thread shape in two halves, against the FP64 oracle."""
+    c = small_case(n=256, nz=4, binning=2, npos=264, seed=17, spread=40.0)
+    _upload_case(mset, c)
+    dV, dp, dpos, err = mset.grad()
+    a, b = mset.grad(0, 132), mset.grad(132, 132)
+    odV, odp, odpos, oep = O.grad_fixed(c['V'], c['probe'], c['prop'], c['bprop'], c['xl'], c['yl'], c['M'],
+                                        c['bin'], c['sig'], c['dx'])
+    print('case: 264 positions at 256: dV', rel(dV, odV), 'dprobe', rel(dp, odp), 'dpos', rel(dpos, odpos), flush=True)
+    assert abs(err / (oep.sum() / c['npos']) - 1) < LOSS_TOL
+    assert rel(dV, odV) < GRAD_TOL and rel(dp, odp) < GRAD_TOL and rel(dpos, odpos) < GRAD_TOL
+    assert rel(a[0] + b[0], dV) < 1e-5 and rel(a[1] + b[1], dp) < 1e-5
+    assert rel(np.r_[a[2], b[2]], dpos) < 1e-6
+
+
 def test_upload_from_page_locked_host_memory(mset):
     """Patterns staged in mset_host_alloc memory upload to the same device state as pageable ones."""
     from mset_amd import lib
