@@ -428,11 +428,46 @@ int read_errors(mset_ctx* c, int first, int count, double* out) {
     return MSET_OK;
 }
 
+// The sequential path on the separable 256 routine: each position is a batched launch of that one position
+// (the 1024-thread shape, one workgroup) with its gradient planes, probe term and position terms, and the
+// host applies the three steps before the next position's transmission and fftn(probe).  The semantics of
+// the one-position kernel below, at one plane pass per propagation step instead of two.
+int issue_plain_sep(mset_ctx* c) {
+    Workspace& W = *c->ws;
+    CHK(W.pslab.ensure(plane(c), "probe term"));
+    CHK(c->dpos.ensure((size_t)2 * c->npos, "position terms"));
+    PosArgs A = base_args(c);
+    A.sep = 1;
+    A.waves = WAVES_STORE;
+    A.count = 1;
+    A.nslab = 1;
+    const float* s = c->p.step_size;
+    for (int k = 0; k < c->npos; ++k) {
+        HIPCHK(launch_transmission_sep(c->V.p, (float*)W.trans.p, (float*)W.trans.p + plane(c) * c->L, c->n, c->nz,
+                                       binning(c), c->p.sigma, c->p.potential_pixelsize, c->stream));
+        HIPCHK(launch_plane_fft(c->n, c->probe.p, W.probeF.p, c->tw.p, 0, 1.f, 0.f, W.planes.p, c->stream));
+        A.first = k;
+        {
+            Timed t(c, "position_plain");
+            HIPCHK(launch_batch(c->n, A, 1, 2, c->stream));
+        }
+        // the gradient planes (odd planes transposed by the routine), then the steps (STEP03_BACK.m:84-92)
+        HIPCHK(launch_untranspose_odd(W.gslab.p, c->n, c->L, c->stream));
+        if (s[0] != 0.f) HIPCHK(launch_update_volume(c->V.p, W.gslab.p, c->n, c->nz, binning(c), s[0], c->stream));
+        if (s[1] != 0.f)
+            HIPCHK(launch_plane_fft(c->n, W.pslab.p, c->probe.p, c->tw.p, 1, -s[1], 1.f, W.planes.p, c->stream));
+        if (s[2] != 0.f)
+            HIPCHK(launch_update_positions(c->x.p + k, c->y.p + k, c->dpos.p + 2 * (size_t)k, 1, s[2], c->stream));
+    }
+    return MSET_OK;
+}
+
 // one launch per position, sequential updates (issue only; no host synchronisation)
 int issue_plain(mset_ctx* c) {
     CHK(ensure_fixed_buffers(c));
     CHK(c->ws->save.ensure((size_t)(c->L + 2) * plane(c), "save waves"));
     CHK(c->ws->gslab.ensure((size_t)c->L * plane(c), "gradient planes"));
+    if (sep_fwd(c) && c->p.grad_mode == GRAD_REF) return issue_plain_sep(c);
     PosArgs A = base_args(c);
     const float s1 = c->p.step_size[0];
     for (int k = 0; k < c->npos; ++k) {
