@@ -5,12 +5,14 @@
 
 namespace mset {
 
-hipError_t launch_batch_variant(int n, const PosArgs& A, int grid, hipStream_t st);
+hipError_t launch_batch_variant(int n, const PosArgs& A, int grid, int shape, hipStream_t st);
 
 hipError_t launch_batch(int n, const PosArgs& A, int grid, int shape, hipStream_t st) {
-    if (shape != 0 && (shape > 2 || A.waves != WAVES_STORE)) return hipErrorInvalidValue;
-    if (shape == 2 && (n != 256 || !A.sep)) return hipErrorInvalidValue;   // the wide shape: separable 256 only
-    if (A.waves != WAVES_STORE) return launch_batch_variant(n, A, grid, st);
+    // shape 1: stored waves; shape 2 (1024 threads): the separable 256 routine, stored or checkpointed waves
+    if (shape == 1 && A.waves != WAVES_STORE) return hipErrorInvalidValue;
+    if (shape == 2 && (n != 256 || !A.sep)) return hipErrorInvalidValue;
+    if (shape < 0 || shape > 2) return hipErrorInvalidValue;
+    if (A.waves != WAVES_STORE) return launch_batch_variant(n, A, grid, shape, st);
     if (A.sep) {
         if (n != 256 || A.grad_mode != GRAD_REF || A.sset) return hipErrorInvalidValue;
         if (shape == 2) hipLaunchKernelGGL((k_batch<256, 2, WAVES_STORE, true>), dim3(grid), dim3(Cfg<256, 2>::NT), 0, st, A);

@@ -316,13 +316,14 @@ int grad_slabs(mset_ctx* c) { return grad_slabs(c, c->ws->shape); }
 
 // grid of a batched launch of `count` positions on S slabs: at most `want` workgroups and, when slabs take
 // several positions (count > S), a divisor of S, so that each slab's positions run on one workgroup in order
-// The workgroup shape of a batched launch: a launch of the separable 256 routine with stored waves and at
-// most one position per CU (a tilt's last mini-batch) runs the 1024-thread shape, one workgroup of 16
-// waves per CU instead of one or two of 8 (15-20 % shorter at 64-256 positions).  Its line transforms,
-// operand order and slab assignment are the default shape's, so the gradient slabs are the same bit for
-// bit; only the workgroup sums (loss, position terms) add their terms in another order.
+// The workgroup shape of a batched launch: a launch of the separable 256 routine (stored or checkpointed
+// waves) with at most one position per CU (a tilt's last mini-batch) runs the 1024-thread shape, one
+// workgroup of 16 waves per CU instead of one or two of 8 (15-20 % shorter at 64-256 positions).  Its line
+// transforms, operand order and slab assignment are the default shape's, so the gradient slabs are the
+// same bit for bit; only the workgroup sums (loss, position terms) add their terms in another order.
 int batch_shape(const mset_ctx* c, int shape, bool sep, int waves, int count) {
-    return c->n == 256 && shape == 0 && sep && waves == WAVES_STORE && count <= c->cus ? 2 : shape;
+    const bool wide = waves == WAVES_STORE || waves == WAVES_CHECKPOINT;
+    return c->n == 256 && shape == 0 && sep && wide && count <= c->cus ? 2 : shape;
 }
 
 int launch_grid(int want, int count, int S) {

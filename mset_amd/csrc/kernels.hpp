@@ -83,8 +83,8 @@ bool plane_size_supported(int n);
 int position_threads(int n);
 hipError_t launch_plain(int n, const PosArgs& A, int k, hipStream_t st);
 // shape: the workgroup shape of the batched kernel (0, or 1 = the alternative shape, stored waves only)
-// shape: 0 default, 1 alternative (MSET_ENGINE_ALT_SHAPE), 2 the separable 256 routine's 1024-thread shape
-// for launches of at most one position per CU
+// shape: 0 default, 1 alternative (MSET_ENGINE_ALT_SHAPE, stored waves), 2 the separable 256 routine's
+// 1024-thread shape for launches of at most one position per CU (stored or checkpointed waves)
 hipError_t launch_batch(int n, const PosArgs& A, int grid, int shape, hipStream_t st);
 hipError_t launch_fwd(int n, const PosArgs& A, int grid, hipStream_t st);
 // forward + loss over the work items A.order[0 .. A.count) of several tilts (A.tilts)

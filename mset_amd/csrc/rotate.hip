@@ -410,6 +410,7 @@ bool shear_groups(const ShearPlan& P, std::vector<LineGroup>& out) {
 // With the shear axes (z, x, y, z) of a tilt about y, a step takes 7 passes instead of 10.
 std::vector<AxisOp> order_ops(const std::vector<LineGroup>& G) {
     const int ng = (int)G.size();
+    if (ng == 0) return {};
     static const int perm2[2][2] = {{0, 1}, {1, 0}};
     static const int perm3[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
     auto nperm = [&](int g) { return G[g].n == 1 ? 1 : G[g].n == 2 ? 2 : 6; };
