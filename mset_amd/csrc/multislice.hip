@@ -24,7 +24,10 @@ int position_threads(int n) {
 }
 
 hipError_t launch_plain(int n, const PosArgs& A, int k, hipStream_t st) {
-    MSET_DISPATCH(n, hipLaunchKernelGGL(k_plain<NN>, dim3(1), dim3(Cfg<NN>::NT), 0, st, A, k));
+    if (A.pfac && n != 256 && !A.sset)
+        MSET_DISPATCH_SMALL(n, hipLaunchKernelGGL((k_plain<NN, true>), dim3(1), dim3(Cfg<NN>::NT), 0, st, A, k))
+    else
+        MSET_DISPATCH(n, hipLaunchKernelGGL((k_plain<NN, false>), dim3(1), dim3(Cfg<NN>::NT), 0, st, A, k));
     return hipGetLastError();
 }
 

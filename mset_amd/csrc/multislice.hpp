@@ -104,7 +104,7 @@ struct AdjPh {
 // slot: the workgroup's scratch (waves, plane buffers); slab: the gradient slab the position adds to;
 // first: the slab's first position of the launch, which writes the slab instead of adding to it (so the
 // slabs need no clearing launch).  PFAC: the propagator is applied as the product of its 1D factors
-// pxs[kx] pys[ky] (in LDS) instead of being read from the 2D array (batched gradient, separable propagator).
+// pxs[kx] pys[ky] (in LDS) instead of being read from the 2D array (separable propagator, below 256).
 template <int N, int NT, int KIND, int WV = WAVES_STORE, bool PFAC = false, class Eng>
 __device__ __forceinline__ void process_position(const PosArgs& A, int k, int slot, int slab, bool first, Eng& E,
                                                  float2* phx, float2* phy, double* red,
@@ -825,11 +825,21 @@ template <int N, int NT> __device__ __forceinline__ void load_twiddles(float2* t
     Eng E;                                                       \
     E.init(s, tw);
 
-template <int N>
+// PFAC: the propagator as its 1D factors from LDS (a separable propagator), as in the batched kernel
+template <int N, bool PFAC = false>
 __global__ __launch_bounds__(Cfg<N>::NT, Cfg<N>::MINB) void k_plain(PosArgs A, int k) {
     constexpr int NT = Cfg<N>::NT;
     MSET_LDS(N, NT, 0)
-    process_position<N, NT, K_PLAIN>(A, k, 0, 0, true, E, phx, phy, red);
+    float2* pxs = nullptr;
+    float2* pys = nullptr;
+    if constexpr (PFAC) {
+        __shared__ float2 fx[N], fy[N];
+        for (int i = threadIdx.x; i < 2 * N; i += NT) (i < N ? fx : fy)[i % N] = A.p1d[i];
+        __syncthreads();
+        pxs = fx;
+        pys = fy;
+    }
+    process_position<N, NT, K_PLAIN, WAVES_STORE, PFAC>(A, k, 0, 0, true, E, phx, phy, red, pxs, pys);
 }
 
 // WV: the exit-wave variant (A.waves), a separate instantiation each so that the store variant carries
