@@ -8,8 +8,8 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SOURCES = ['batch_store.hip', 'batch_variants.hip', 'multislice.hip', 'elementwise.hip', 'rotate.hip', 'tv.hip',
-           'capi.hip']
+SOURCES = ['batch_store.hip', 'batch_variants.hip', 'multislice.hip', 'forward.hip', 'elementwise.hip', 'rotate.hip',
+           'tv.hip', 'capi.hip']
 LIB = os.path.join(HERE, 'libmset_amd.so')
 # stand-alone counter-calibration program (known-size copies for rocprofv3's FETCH_SIZE / WRITE_SIZE)
 CALIBRATE = os.path.join(HERE, 'mset_calibrate')

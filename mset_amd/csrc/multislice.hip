@@ -1,4 +1,4 @@
-// multislice.hip -- launchers of the sequential, forward-only and plane-FFT kernels (multislice.hpp).
+// multislice.hip -- launchers of the sequential and plane-FFT kernels (multislice.hpp).
 #include "multislice.hpp"
 
 namespace mset {
@@ -31,22 +31,6 @@ hipError_t launch_plain(int n, const PosArgs& A, int k, hipStream_t st) {
     return hipGetLastError();
 }
 
-
-hipError_t launch_fwd(int n, const PosArgs& A, int grid, hipStream_t st) {
-    if (A.sep) {
-        if (n != 256 || A.sset) return hipErrorInvalidValue;
-        hipLaunchKernelGGL((k_fwd<256, false, true>), dim3(grid), dim3(Cfg<256>::NT), 0, st, A);
-    } else MSET_DISPATCH(n, hipLaunchKernelGGL((k_fwd<NN, false, false>), dim3(grid), dim3(Cfg<NN>::NT), 0, st, A));
-    return hipGetLastError();
-}
-
-hipError_t launch_fwd_tilts(int n, const PosArgs& A, int grid, hipStream_t st) {
-    if (A.sep) {
-        if (n != 256 || A.sset) return hipErrorInvalidValue;
-        hipLaunchKernelGGL((k_fwd<256, true, true>), dim3(grid), dim3(Cfg<256>::NT), 0, st, A);
-    } else MSET_DISPATCH(n, hipLaunchKernelGGL((k_fwd<NN, true, false>), dim3(grid), dim3(Cfg<NN>::NT), 0, st, A));
-    return hipGetLastError();
-}
 
 hipError_t launch_plane_fft(int n, const float2* in, float2* out, const float2* tw, int dir, float alpha, float beta,
                             float2* planes, hipStream_t st, float2* out2, float alpha2, float beta2) {

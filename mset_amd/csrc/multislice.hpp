@@ -882,13 +882,14 @@ __global__ __launch_bounds__((Cfg<N, SH>::NT), (Cfg<N, SH>::MINB)) void k_batch(
 
 // MT: the work items are positions of several tilts (A.tilts, A.order), so a whole error pass over a tilt
 // series is one persistent launch; a separate instantiation, so the one-tilt kernel carries none of it
-template <int N, bool MT, bool SEP>
+// PFAC (general routine below 256): the propagator as its 1D factors from LDS.
+template <int N, bool MT, bool SEP, bool PFAC = false>
 __global__ __launch_bounds__(Cfg<N>::NT, Cfg<N>::MINB) void k_fwd(PosArgs A) {
     constexpr int NT = Cfg<N>::NT;
     MSET_LDS(N, NT, 0)
     float2* px = nullptr;
     float2* py = nullptr;
-    if constexpr (SEP) {
+    if constexpr (SEP || PFAC) {
         __shared__ float2 pxs[N], pys[N];
         for (int i = threadIdx.x; i < 2 * N; i += NT) (i < N ? pxs : pys)[i % N] = A.p1d[i];
         __syncthreads();
@@ -909,11 +910,13 @@ __global__ __launch_bounds__(Cfg<N>::NT, Cfg<N>::MINB) void k_fwd(PosArgs A) {
             B.err_pos = T.err_pos;
             B.obase = 0;
             if constexpr (SEP) process_sep<N, NT, K_FWD>(B, item & ITEM_POS_MASK, blockIdx.x, blockIdx.x, true, E, phx, phy, px, py, red);
-            else process_position<N, NT, K_FWD>(B, item & ITEM_POS_MASK, blockIdx.x, blockIdx.x, true, E, phx, phy, red);
+            else process_position<N, NT, K_FWD, WAVES_STORE, PFAC>(B, item & ITEM_POS_MASK, blockIdx.x, blockIdx.x, true, E,
+                                                                  phx, phy, red, px, py);
         } else {
             const int k = A.order ? A.order[A.first + p] : A.first + p;
             if constexpr (SEP) process_sep<N, NT, K_FWD>(A, k, blockIdx.x, blockIdx.x, true, E, phx, phy, px, py, red);
-            else process_position<N, NT, K_FWD>(A, k, blockIdx.x, blockIdx.x, p == (int)blockIdx.x, E, phx, phy, red);
+            else process_position<N, NT, K_FWD, WAVES_STORE, PFAC>(A, k, blockIdx.x, blockIdx.x, p == (int)blockIdx.x, E,
+                                                                  phx, phy, red, px, py);
         }
         __syncthreads();
     }
