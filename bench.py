@@ -1,12 +1,13 @@
 """Benchmark of the multislice forward + adjoint path on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--workload C3|C2] [--batch B] [--no-autotune] [--gpus 1]
+    python bench.py [--steps K] [--warmup W] [--workload C3|C2|C1] [--batch B] [--no-autotune] [--gpus 1]
 
 Default workload (C3, the largest single-GPU configuration): a 128^3 Gaussian-atom volume at 0.4 A in a
 256 x 256 field (256 x 256 x 128 voxels), probe 256 x 256 at 300 kV / 21 mrad, 21 tilts from -60 to +60
 deg, each tilt with its own square scan of side drawn from 24..48 (2 px steps + U[-0.5, 0.5] px jitter),
 1.6 A slices -> 32 propagation planes, Poisson data at 5e4 e/A^2; synthetic and seeded.  `--workload C2`
-runs the 128 x 128 configuration (64^3 volume, 32 x 32 scans, 11 tilts, 16 planes).
+runs the 128 x 128 configuration (64^3 volume, 32 x 32 scans, 11 tilts, 16 planes), `--workload C1` the small
+CPU-runnable case (24^3 volume in a 64 x 64 field, 8 x 8 scans, 3 tilts, 6 planes; its CPU sample is a whole tilt).
 
 One step = one full update pass of the reconstruction over every tilt (Run_MSET_recon.m:119-187): for
 each tilt the inverse Fourier-shear rotation of the volume, forward + adjoint for all of the tilt's
@@ -35,6 +36,8 @@ METRIC = 'probe-position·slices/sec per fwd+adjoint iteration; HBM GB/s; p50/p9
 UNIT = 'probe-position·slices/s'
 HBM_PEAK_GBS = 8000.0
 WORKLOADS = {
+    'C1': 'C1: 24^3 volume at 0.5 A padded to 64x64x24, probe 64x64 300 kV 25 mrad, 8x8 scan at 0.8 A, 3 tilts '
+          '-30/0/30 deg, 6 planes (2 A), one update pass over all tilts per step',
     'C2': 'C2: 64^3 volume at 0.4 A padded to 128x128x64, probe 128x128 300 kV 21 mrad, 32x32 scan '
           '(2 px + U[-0.5,0.5]), 11 tilts -50..50 deg, 16 planes (1.6 A), one update pass over all tilts per step',
     'C3': 'C3: 128^3 volume at 0.4 A padded to 256x256x128, probe 256x256 300 kV 21 mrad, 21 tilts -60..60 deg, '
@@ -88,8 +91,9 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=3)
-    ap.add_argument('--workload', default='C3', choices=['C2', 'C3'],
-                    help='C3 (default): 256^2 probe, 21 tilts with uneven scans; C2: 128^2 probe, 11 tilts')
+    ap.add_argument('--workload', default='C3', choices=['C1', 'C2', 'C3'],
+                    help='C3 (default): 256^2 probe, 21 tilts with uneven scans; C2: 128^2 probe, 11 tilts; C1: the '
+                         'small CPU-runnable case (64^2 probe, 3 tilts of 64 positions)')
     ap.add_argument('--batch', type=int, default=None,
                     help='positions per mini-batch (default: picked by the autotuner from 256 and 512 for C3, '
                          'the range of batch sizes that workload names, and from 512 and 1024 for C2)')
@@ -114,13 +118,15 @@ def main():
 
     from mset_amd import lib, physics, synth
     big = args.workload == 'C3'
-    batches = (256, 512) if big else (512, 1024)
+    small = args.workload == 'C1'
+    batches = (256, 512) if big else (16, 64) if small else (512, 1024)
     if args.batch is not None:
         batches = (args.batch,)
     elif args.no_autotune:
-        batches = (512 if big else 1024,)
+        batches = (batches[-1],)
     if args.cpu_positions is None:
-        args.cpu_positions = 120 if big else 1024
+        # C1 in full (a whole tilt of 64 positions); C2 and C3 a sample of about 10 s
+        args.cpu_positions = 120 if big else 64 if small else 1024
     if args.cpu_threads is None:
         args.cpu_threads = host_threads()
 
